@@ -1,0 +1,64 @@
+"""Oracle clover observables (GLAC/observables/mastersamplertopcxyz.cpp:236-384).
+
+Pins: unit field; constant abelian fields whose plaquette, energy and charge
+have closed forms (every leaf equals diag(e^{i phi}, e^{-i phi}, 1), so
+C - C^dagger = diag(8i sin phi, -8i sin phi, 0) and
+Q = V sin(phi1) sin(phi2) / (2 pi^2) -> 2 k01 k23 as the lattice grows);
+gauge invariance; time reflection flips Q and keeps E; sum_t Q(t) = Q
+(scripts/topct_sanity_test/topct_topc_test.py:31-37)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from fieldtools import abelian_field, gauge_transform, plaquettes, random_field, random_su3, reflect_t
+
+
+def test_unit_field():
+    n = (4, 4, 4, 8)
+    plaq, e, q, qt = O.measure(O.unit_field(n), n)
+    assert plaq == 1.0 and e == 0.0 and q == 0.0 and np.all(qt == 0.0)
+
+
+@pytest.mark.parametrize("n,k1,k2", [((6, 6, 4, 4), 1, 1), ((8, 8, 6, 6), 1, -2), ((8, 4, 6, 8), 2, 1)])
+def test_constant_abelian_field_closed_forms(n, k1, k2):
+    U, phi1, phi2 = abelian_field(n, k1, k2)
+    V = int(np.prod(n))
+    plaq, e, q, qt = O.measure(U, n)
+    s1, s2 = np.sin(phi1), np.sin(phi2)
+    assert abs(plaq - ((2 * np.cos(phi1) + 1) + (2 * np.cos(phi2) + 1) + 12) / 18) < 1e-14
+    assert abs(e - (-128.0 * (s1 * s1 + s2 * s2))) < 1e-11
+    assert abs(q - V * s1 * s2 / (2 * np.pi ** 2)) < 1e-12
+    assert np.allclose(qt, q / n[3], atol=1e-13)
+    # continuum limit: Q -> 2 k01 k23
+    assert abs(q - 2 * k1 * k2) < 0.25 * abs(2 * k1 * k2)
+
+
+def test_plaquette_matches_brute_force():
+    rng = np.random.default_rng(31)
+    n = (4, 4, 6, 4)
+    U = random_field(n, rng)
+    plaq = O.measure(U, n)[0]
+    assert abs(plaq - plaquettes(U, n).sum() / (18 * np.prod(n))) < 1e-14
+
+
+def test_gauge_invariance_and_sum_rule():
+    rng = np.random.default_rng(32)
+    n = (4, 4, 4, 6)
+    U = random_field(n, rng)
+    g = random_su3(rng, (n[3], n[2], n[1], n[0]))
+    a = O.measure(U, n)
+    b = O.measure(gauge_transform(U, n, g), n)
+    assert abs(a[0] - b[0]) < 1e-14 and abs(a[1] - b[1]) < 1e-11 and abs(a[2] - b[2]) < 1e-13
+    assert np.abs(a[3] - b[3]).max() < 1e-13
+    assert abs(a[3].sum() - a[2]) < 1e-14
+
+
+def test_time_reflection_flips_charge():
+    rng = np.random.default_rng(33)
+    n = (4, 4, 4, 6)
+    U = random_field(n, rng)
+    a = O.measure(U, n)
+    b = O.measure(reflect_t(U, n), n)
+    assert abs(a[0] - b[0]) < 1e-14 and abs(a[1] - b[1]) < 1e-11
+    assert abs(a[2] + b[2]) < 1e-13
+    assert abs(a[2]) > 1e-4  # the check is not vacuous
