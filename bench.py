@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Benchmark of the GLAC hot path on one MI355X.
+
+Workload: a 32^4 periodic lattice at beta = 6.1 (the lattice of the headline
+metric).  A *step* is one RK3 Wilson-flow step (eps = 0.01) with FP32 links,
+FP64 exponential coefficients and FP64 reductions, followed by the clover
+measurement (plaquette, E, Q, Q(t)) -- one iteration of GLAC's
+System::flowConfiguration loop (GLAC/system.cpp:494-508).  ``value`` is flow
+steps per second with the field resident in HBM.
+
+Also reported, each from its own timed region on the same lattice:
+ - Metropolis sweeps/s (FP32 links, 10 hits, eps = 0.24, reunitarised every
+   sweep) and FP64 flow steps/s;
+ - ``roofline``: the flow-stage kernel's algorithmic bytes per launch over its
+   mean launch time (HIP events on the library's stream), against 8 TB/s;
+ - ``cpu_baseline``: the FP64 oracle (one host thread) doing one flow step plus
+   one measurement on the same 32^4 lattice.
+
+Run ``python bench.py [--steps K] [--warmup W]``; one JSON line on stdout.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "indep. configs/s (tau_int Q^2) + flow steps/s at 32^4, FP32/learned vs FP64 plain"
+
+
+def link_bytes(fp32):
+    return 72 if fp32 else 144      # 18 reals per link
+
+
+def alg_bytes(fp32):
+    return 32 if fp32 else 64       # 8 reals per su(3) accumulator entry
+
+
+def flow_stage_bytes_per_launch(V, fp32):
+    """Per RK3 step the three stage launches read 4V links and write 4V links
+    each, and move the su(3) accumulator 4V entries (write), 8V (read+write),
+    4V (read): 24 s_L + 16 s_Z per site per step, a third per launch."""
+    return (24 * link_bytes(fp32) + 16 * alg_bytes(fp32)) * V / 3.0
+
+
+def metropolis_bytes_per_launch(V, fp32):
+    """A sub-sweep (one mu, one parity) reads all 4V links and writes V/2."""
+    return 4.5 * V * link_bytes(fp32)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--dims", default="32,32,32,32")
+    ap.add_argument("--sweeps", type=int, default=100, help="timed Metropolis sweeps")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the lattice shards by 4D domain decomposition, which is not built here).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import glac_amd as G
+
+    n = tuple(int(v) for v in args.dims.split(","))
+    V = int(np.prod(n))
+    beta, eps_flow = 6.1, 0.01
+
+    # ---------------------------------------------------------------- FP32 flow
+    L = G.Lattice(n, G.FP32_LINKS)
+    L.hot_start(20261202)
+    log(f"[bench] lattice {n}, FP32 links, {L.device_bytes / 2**20:.0f} MiB on device")
+    # Metropolis: a few thermalising sweeps, then the timed sweeps
+    L.metropolis(5, beta, 0.24, 10, reunit_every=1, seed=7, sweep_offset=0)
+    L.synchronize()
+    L.timing(True)
+    L.timing_reset()
+    t0 = time.perf_counter()
+    acc = L.metropolis(args.sweeps, beta, 0.24, 10, reunit_every=1, seed=7, sweep_offset=5)
+    L.synchronize()
+    t_sweeps = time.perf_counter() - t0
+    m_launches, m_ms = L.timing_read(G.K_METROPOLIS)
+    L.timing(False)
+    sweeps_per_s = args.sweeps / t_sweeps
+    log(f"[bench] metropolis {args.sweeps} sweeps: {sweeps_per_s:.1f} sweeps/s, "
+        f"acceptance {acc.mean():.4f}")
+
+    L.copy(G.FLOW, G.MAIN)
+    L.flow(args.warmup, eps_flow, field=G.FLOW, measure=True)
+    L.synchronize()
+    L.timing(True)
+    L.timing_reset()
+    t0 = time.perf_counter()
+    plaq, e_raw, q, qt = L.flow(args.steps, eps_flow, field=G.FLOW, measure=True)
+    L.synchronize()
+    elapsed = time.perf_counter() - t0
+    f_launches, f_ms = L.timing_read(G.K_FLOW_STAGE)
+    meas_launches, meas_ms = L.timing_read(G.K_MEASURE)
+    L.timing(False)
+    L.close()
+    steps_per_s = args.steps / elapsed
+    log(f"[bench] flow fp32: {args.steps} steps in {elapsed:.3f} s = {steps_per_s:.1f} steps/s; "
+        f"stage kernel {f_ms / max(f_launches, 1):.4f} ms x {f_launches}, "
+        f"measure {meas_ms / max(meas_launches, 1):.4f} ms x {meas_launches}")
+
+    stage_bytes = flow_stage_bytes_per_launch(V, True)
+    stage_ms = f_ms / max(f_launches, 1)
+    achieved = stage_bytes / (stage_ms * 1e-3) / 1e9
+    metro_bytes = metropolis_bytes_per_launch(V, True)
+    metro_ms = m_ms / max(m_launches, 1)
+    metro_gbs = metro_bytes / (metro_ms * 1e-3) / 1e9
+
+    # ---------------------------------------------------------------- FP64 flow
+    fp64 = None
+    if not args.no_fp64:
+        L = G.Lattice(n, G.FP64)
+        L.hot_start(20261202, field=G.FLOW)
+        k64 = max(args.steps // 3, 10)
+        L.flow(min(args.warmup, 10), eps_flow, field=G.FLOW)
+        L.synchronize()
+        L.timing(True)
+        L.timing_reset()
+        t0 = time.perf_counter()
+        L.flow(k64, eps_flow, field=G.FLOW)
+        L.synchronize()
+        el64 = time.perf_counter() - t0
+        n64, ms64 = L.timing_read(G.K_FLOW_STAGE)
+        L.close()
+        ms_stage64 = ms64 / max(n64, 1)
+        fp64 = {
+            "flow_steps_per_s": k64 / el64,
+            "stage_kernel_ms": ms_stage64,
+            "stage_kernel_gbs": flow_stage_bytes_per_launch(V, False) / (ms_stage64 * 1e-3) / 1e9,
+        }
+        log(f"[bench] flow fp64: {fp64['flow_steps_per_s']:.1f} steps/s, stage kernel {ms_stage64:.4f} ms")
+
+    # ---------------------------------------------------------------- CPU baseline
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle as O
+        U = O.hot_start_haar(n, 1)
+        t0 = time.perf_counter()
+        O.flow_step(U, n, eps_flow)
+        O.measure(U, n)
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": 1.0 / cpu_s, "unit": "flow steps/s", "cores": 1, "kind": "port",
+               "sample": f"one RK3 step + one clover measurement of the FP64 oracle on the full "
+                         f"{'x'.join(map(str, n))} lattice, one host thread ({cpu_s:.1f} s)"}
+        log(f"[bench] cpu baseline: {cpu_s:.2f} s per step")
+
+    out = {
+        "metric": METRIC,
+        "value": steps_per_s,
+        "unit": "flow steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 links, fp64 exp coefficients and sums",
+        "data": "synthetic: Haar hot start + 105 Metropolis sweeps at beta 6.1, seed 20261202",
+        "config": {"workload": "RK3 Wilson flow step + clover measurement", "lattice": list(n),
+                   "beta": beta, "flow_eps": eps_flow, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "flow_stage_kernel<float>",
+                     "alg_bytes_per_launch": stage_bytes, "mean_launch_ms": stage_ms},
+        "metropolis": {"sweeps_per_s": sweeps_per_s, "acceptance": float(acc.mean()),
+                       "subsweep_kernel_ms": metro_ms, "subsweep_kernel_gbs": metro_gbs,
+                       "hbm_frac": metro_gbs / HBM_PEAK_GBS},
+        "measure_kernel_ms": meas_ms / max(meas_launches, 1),
+        "fp64_plain": fp64,
+        "indep_configs_per_s": None,
+        "cpu_baseline": cpu,
+        "final_obs": {"plaq": float(plaq[-1]), "E": float(-e_raw[-1] / 64), "Q": float(q[-1])},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

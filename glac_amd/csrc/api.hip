@@ -1,0 +1,525 @@
+// api.hip -- the C ABI of libglacx.so (declared in include/glacx.h): context,
+// device memory, host transfers, .bin I/O and the sweep / flow / measurement
+// drivers that sequence the kernels on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/glacx.h"
+#include "kernels.hpp"
+
+using namespace glacx;
+
+struct glacx_ctx {
+    int dims[4] = {0, 0, 0, 0};
+    Geom geom{};
+    int precision = GLACX_FP64;
+    int device = 0;
+    size_t real_size = 8;
+    hipStream_t stream = nullptr;
+    void *field[2] = {nullptr, nullptr};
+    void *work = nullptr;            // flow ping-pong buffer (one field)
+    void *acc = nullptr;             // flow accumulator, 32 reals per site
+    double *partials = nullptr;      // reduction partials
+    size_t partials_len = 0;
+    double *results = nullptr;       // device results (measurement rows)
+    size_t results_len = 0;
+    unsigned long long *counts = nullptr;  // accepted hits per sweep
+    size_t counts_len = 0;
+    double *stage = nullptr;         // AoS FP64 staging for host transfers
+    size_t device_bytes = 0;
+    std::string err;
+    // per-kernel event timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> timed;
+    long long t_launches[GLACX_K_COUNT] = {0, 0, 0, 0};
+    double t_ms[GLACX_K_COUNT] = {0, 0, 0, 0};
+};
+
+namespace {
+
+int fail(glacx_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, call)                                                                              \
+    do {                                                                                                 \
+        hipError_t e_ = (call);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            return fail((ctx), GLACX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+#define CHECK_LAUNCH(ctx) HIP_TRY(ctx, hipGetLastError())
+
+size_t field_bytes(const glacx_ctx *c) { return (size_t)36 * c->geom.v * 2 * c->real_size; }
+
+bool valid_field(int f) { return f == GLACX_FIELD_MAIN || f == GLACX_FIELD_FLOW; }
+
+int dev_alloc(glacx_ctx *c, void **p, size_t bytes)
+{
+    HIP_TRY(c, hipMalloc(p, bytes));
+    c->device_bytes += bytes;
+    return GLACX_OK;
+}
+
+// Grow a device buffer of doubles to at least n entries.
+template <class P> int ensure(glacx_ctx *c, P **buf, size_t *len, size_t n)
+{
+    if (*len >= n) return GLACX_OK;
+    if (*buf) {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        HIP_TRY(c, hipFree(*buf));
+        c->device_bytes -= *len * sizeof(P);
+        *buf = nullptr;
+        *len = 0;
+    }
+    void *p = nullptr;
+    int rc = dev_alloc(c, &p, n * sizeof(P));
+    if (rc) return rc;
+    *buf = (P *)p;
+    *len = n;
+    return GLACX_OK;
+}
+
+int ensure_stage(glacx_ctx *c)
+{
+    if (c->stage) return GLACX_OK;
+    void *p = nullptr;
+    int rc = dev_alloc(c, &p, (size_t)4 * c->geom.v * 18 * sizeof(double));
+    c->stage = (double *)p;
+    return rc;
+}
+
+// ---- event timing --------------------------------------------------------
+int timed_begin(glacx_ctx *c, hipEvent_t *start)
+{
+    if (!c->timing) return GLACX_OK;
+    if (c->ev_used + 2 > c->ev_pool.size()) {
+        for (int k = 0; k < 64; k++) {
+            hipEvent_t e;
+            HIP_TRY(c, hipEventCreate(&e));
+            c->ev_pool.push_back(e);
+        }
+    }
+    *start = c->ev_pool[c->ev_used++];
+    HIP_TRY(c, hipEventRecord(*start, c->stream));
+    return GLACX_OK;
+}
+
+int timed_end(glacx_ctx *c, int kernel, hipEvent_t start)
+{
+    if (!c->timing) return GLACX_OK;
+    hipEvent_t stop = c->ev_pool[c->ev_used++];
+    HIP_TRY(c, hipEventRecord(stop, c->stream));
+    c->timed.push_back({kernel, {start, stop}});
+    // keep the pool bounded: fold finished pairs into the totals
+    if (c->timed.size() >= 4096) {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        for (auto &t : c->timed) {
+            float ms = 0.f;
+            HIP_TRY(c, hipEventElapsedTime(&ms, t.second.first, t.second.second));
+            c->t_launches[t.first]++;
+            c->t_ms[t.first] += ms;
+        }
+        c->timed.clear();
+        c->ev_used = 0;
+    }
+    return GLACX_OK;
+}
+
+// ---- kernel sequences ----------------------------------------------------
+template <class T> int run_measure(glacx_ctx *c, const void *f, double *out)
+{
+    int rc = ensure(c, &c->partials, &c->partials_len, (size_t)measure_partials_count(c->geom));
+    if (rc) return rc;
+    hipEvent_t e0 = nullptr;
+    if ((rc = timed_begin(c, &e0))) return rc;
+    launch_measure<T>(f, c->geom, c->partials, c->stream);
+    CHECK_LAUNCH(c);
+    if ((rc = timed_end(c, GLACX_K_MEASURE, e0))) return rc;
+    launch_measure_finish(c->partials, c->geom, out, c->stream);
+    CHECK_LAUNCH(c);
+    return GLACX_OK;
+}
+
+template <class T> int run_plaquette(glacx_ctx *c, const void *f, double *out)
+{
+    const int np = plaquette_partials_count(c->geom);
+    int rc = ensure(c, &c->partials, &c->partials_len, (size_t)np);
+    if (rc) return rc;
+    hipEvent_t e0 = nullptr;
+    if ((rc = timed_begin(c, &e0))) return rc;
+    launch_plaquette<T>(f, c->geom, c->partials, c->stream);
+    CHECK_LAUNCH(c);
+    if ((rc = timed_end(c, GLACX_K_PLAQUETTE, e0))) return rc;
+    launch_plaquette_finish(c->partials, np, c->geom, out, c->stream);
+    CHECK_LAUNCH(c);
+    return GLACX_OK;
+}
+
+// One RK3 step on `f` (result back in `f`): three stages ping-ponging
+// through the work buffer, then a device copy back for the odd stage count.
+template <class T> int run_flow_step(glacx_ctx *c, void *f, double eps)
+{
+    void *bufs[2] = {f, c->work};
+    int rc;
+    for (int s = 0; s < 3; s++) {
+        hipEvent_t e0 = nullptr;
+        if ((rc = timed_begin(c, &e0))) return rc;
+        launch_flow_stage<T>(bufs[s & 1], bufs[(s + 1) & 1], c->acc, c->geom, s, eps, c->stream);
+        CHECK_LAUNCH(c);
+        if ((rc = timed_end(c, GLACX_K_FLOW_STAGE, e0))) return rc;
+    }
+    return GLACX_OK;
+}
+
+template <class T> int run_metropolis_sweep(glacx_ctx *c, const MetroParams &p, unsigned long long *count)
+{
+    int rc;
+    for (int mu = 0; mu < 4; mu++)
+        for (int parity = 0; parity < 2; parity++) {
+            hipEvent_t e0 = nullptr;
+            if ((rc = timed_begin(c, &e0))) return rc;
+            launch_metropolis<T>(c->field[GLACX_FIELD_MAIN], c->geom, mu, parity, p, count, c->stream);
+            CHECK_LAUNCH(c);
+            if ((rc = timed_end(c, GLACX_K_METROPOLIS, e0))) return rc;
+        }
+    return GLACX_OK;
+}
+
+}  // namespace
+
+#define CTX_CHECK(c)                                                                                    \
+    do {                                                                                                 \
+        if (!(c)) return GLACX_E_ARG;                                                                    \
+        HIP_TRY(c, hipSetDevice((c)->device));                                                           \
+    } while (0)
+
+#define DISPATCH(c, fn, ...) ((c)->precision == GLACX_FP32_LINKS ? fn<float>(__VA_ARGS__) : fn<double>(__VA_ARGS__))
+
+extern "C" {
+
+int glacx_create(const int dims[4], int precision, int device, glacx_ctx **out)
+{
+    if (!dims || !out) return GLACX_E_ARG;
+    *out = nullptr;
+    long v = 1;
+    for (int k = 0; k < 4; k++) {
+        if (dims[k] < 2 || (dims[k] & 1)) return GLACX_E_ARG;
+        v *= dims[k];
+    }
+    if (v > (1L << 30)) return GLACX_E_ARG;
+    if (precision != GLACX_FP64 && precision != GLACX_FP32_LINKS) return GLACX_E_ARG;
+    glacx_ctx *c = new glacx_ctx;
+    for (int k = 0; k < 4; k++) c->dims[k] = c->geom.n[k] = dims[k];
+    c->geom.nxh = dims[0] / 2;
+    c->geom.v = (int)v;
+    c->geom.vh = (int)(v / 2);
+    c->geom.vs_h = dims[0] * dims[1] * dims[2] / 2;
+    c->precision = precision;
+    c->real_size = precision == GLACX_FP32_LINKS ? 4 : 8;
+    c->device = device;
+    int rc = GLACX_OK;
+    auto bad = [&](int code) {
+        *out = c;  // keep the context so the caller can read the message
+        return code;
+    };
+    if (hipSetDevice(device) != hipSuccess) return bad(fail(c, GLACX_E_HIP, "hipSetDevice failed"));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bad(fail(c, GLACX_E_HIP, "hipStreamCreate failed"));
+    for (int f = 0; f < 2; f++)
+        if ((rc = dev_alloc(c, &c->field[f], field_bytes(c)))) return bad(rc);
+    if ((rc = dev_alloc(c, &c->work, field_bytes(c)))) return bad(rc);
+    if ((rc = dev_alloc(c, &c->acc, (size_t)32 * v * c->real_size))) return bad(rc);
+    if ((rc = ensure(c, &c->results, &c->results_len, (size_t)(3 + dims[3])))) return bad(rc);
+    *out = c;
+    return GLACX_OK;
+}
+
+int glacx_destroy(glacx_ctx *c)
+{
+    if (!c) return GLACX_E_ARG;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void *p : {c->field[0], c->field[1], c->work, c->acc, (void *)c->partials, (void *)c->results,
+                    (void *)c->counts, (void *)c->stage})
+        if (p) hipFree(p);
+    for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return GLACX_OK;
+}
+
+const char *glacx_last_error(const glacx_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int glacx_device_bytes(const glacx_ctx *c, size_t *bytes)
+{
+    if (!c || !bytes) return GLACX_E_ARG;
+    *bytes = c->device_bytes;
+    return GLACX_OK;
+}
+
+int glacx_synchronize(glacx_ctx *c)
+{
+    CTX_CHECK(c);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GLACX_OK;
+}
+
+int glacx_upload_field(glacx_ctx *c, int field, const double *const links[4])
+{
+    CTX_CHECK(c);
+    if (!valid_field(field) || !links) return fail(c, GLACX_E_ARG, "upload: bad field or pointer");
+    for (int mu = 0; mu < 4; mu++)
+        if (!links[mu]) return fail(c, GLACX_E_ARG, "upload: null link array");
+    int rc = ensure_stage(c);
+    if (rc) return rc;
+    const size_t n = (size_t)c->geom.v * 18;
+    for (int mu = 0; mu < 4; mu++)
+        HIP_TRY(c, hipMemcpyAsync(c->stage + mu * n, links[mu], n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    DISPATCH(c, launch_pack, c->stage, c->field[field], c->geom, 0, c->stream);
+    CHECK_LAUNCH(c);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GLACX_OK;
+}
+
+int glacx_download_field(glacx_ctx *c, int field, double *const links[4])
+{
+    CTX_CHECK(c);
+    if (!valid_field(field) || !links) return fail(c, GLACX_E_ARG, "download: bad field or pointer");
+    for (int mu = 0; mu < 4; mu++)
+        if (!links[mu]) return fail(c, GLACX_E_ARG, "download: null link array");
+    int rc = ensure_stage(c);
+    if (rc) return rc;
+    DISPATCH(c, launch_unpack, c->field[field], c->stage, c->geom, 0, c->stream);
+    CHECK_LAUNCH(c);
+    const size_t n = (size_t)c->geom.v * 18;
+    for (int mu = 0; mu < 4; mu++)
+        HIP_TRY(c, hipMemcpyAsync(links[mu], c->stage + mu * n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GLACX_OK;
+}
+
+int glacx_copy_field(glacx_ctx *c, int dst, int src)
+{
+    CTX_CHECK(c);
+    if (!valid_field(dst) || !valid_field(src)) return fail(c, GLACX_E_ARG, "copy: bad field");
+    if (dst == src) return GLACX_OK;
+    HIP_TRY(c, hipMemcpyAsync(c->field[dst], c->field[src], field_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+    return GLACX_OK;
+}
+
+int glacx_load_bin(glacx_ctx *c, int field, const char *path)
+{
+    CTX_CHECK(c);
+    if (!valid_field(field) || !path) return fail(c, GLACX_E_ARG, "load_bin: bad field or path");
+    const size_t n = (size_t)c->geom.v * 72;
+    FILE *fp = std::fopen(path, "rb");
+    if (!fp) return fail(c, GLACX_E_IO, std::string("load_bin: cannot open ") + path);
+    std::vector<double> buf(n);
+    const size_t got = std::fread(buf.data(), sizeof(double), n, fp);
+    std::fclose(fp);
+    if (got != n) return fail(c, GLACX_E_IO, std::string("load_bin: short file ") + path);
+    int rc = ensure_stage(c);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(c->stage, buf.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    DISPATCH(c, launch_pack, c->stage, c->field[field], c->geom, 1, c->stream);
+    CHECK_LAUNCH(c);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GLACX_OK;
+}
+
+int glacx_save_bin(glacx_ctx *c, int field, const char *path)
+{
+    CTX_CHECK(c);
+    if (!valid_field(field) || !path) return fail(c, GLACX_E_ARG, "save_bin: bad field or path");
+    int rc = ensure_stage(c);
+    if (rc) return rc;
+    const size_t n = (size_t)c->geom.v * 72;
+    std::vector<double> buf(n);
+    DISPATCH(c, launch_unpack, c->field[field], c->stage, c->geom, 1, c->stream);
+    CHECK_LAUNCH(c);
+    HIP_TRY(c, hipMemcpyAsync(buf.data(), c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) return fail(c, GLACX_E_IO, std::string("save_bin: cannot open ") + path);
+    const size_t put = std::fwrite(buf.data(), sizeof(double), n, fp);
+    const int closed = std::fclose(fp);
+    if (put != n || closed != 0) return fail(c, GLACX_E_IO, std::string("save_bin: write failed ") + path);
+    return GLACX_OK;
+}
+
+int glacx_cold_start(glacx_ctx *c, int field)
+{
+    CTX_CHECK(c);
+    if (!valid_field(field)) return fail(c, GLACX_E_ARG, "cold_start: bad field");
+    DISPATCH(c, launch_cold_start, c->field[field], c->geom, c->stream);
+    CHECK_LAUNCH(c);
+    return GLACX_OK;
+}
+
+int glacx_hot_start(glacx_ctx *c, int field, uint64_t seed)
+{
+    CTX_CHECK(c);
+    if (!valid_field(field)) return fail(c, GLACX_E_ARG, "hot_start: bad field");
+    DISPATCH(c, launch_hot_start, c->field[field], c->geom, seed, c->stream);
+    CHECK_LAUNCH(c);
+    return GLACX_OK;
+}
+
+int glacx_metropolis(glacx_ctx *c, int n_sweeps, double beta, double eps, int n_hit, int reunit_every, uint64_t seed,
+                     uint64_t sweep_offset, double *accept, double *plaq)
+{
+    CTX_CHECK(c);
+    if (n_sweeps < 0 || n_hit < 1 || !(eps > 0.0 && eps < 1.0))
+        return fail(c, GLACX_E_ARG, "metropolis: need n_sweeps >= 0, n_hit >= 1, 0 < eps < 1");
+    if (n_sweeps == 0) return GLACX_OK;
+    int rc = ensure(c, &c->counts, &c->counts_len, (size_t)n_sweeps);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemsetAsync(c->counts, 0, sizeof(unsigned long long) * n_sweeps, c->stream));
+    const size_t row = (size_t)3 + c->dims[3];
+    if (plaq && (rc = ensure(c, &c->results, &c->results_len, (size_t)n_sweeps))) return rc;
+    for (int i = 0; i < n_sweeps; i++) {
+        MetroParams p;
+        p.beta = beta;
+        p.eps = eps;
+        p.n_hit = n_hit;
+        p.sweep = sweep_offset + (uint64_t)i;
+        p.reunit = reunit_every > 0 && (p.sweep % (uint64_t)reunit_every) == 0;
+        p.seed = seed;
+        if ((rc = DISPATCH(c, run_metropolis_sweep, c, p, c->counts + i))) return rc;
+        if (plaq && (rc = DISPATCH(c, run_plaquette, c, c->field[GLACX_FIELD_MAIN], c->results + i))) return rc;
+    }
+    (void)row;
+    std::vector<unsigned long long> counts(n_sweeps);
+    HIP_TRY(c, hipMemcpyAsync(counts.data(), c->counts, sizeof(unsigned long long) * n_sweeps, hipMemcpyDeviceToHost,
+                              c->stream));
+    if (plaq) HIP_TRY(c, hipMemcpyAsync(plaq, c->results, sizeof(double) * n_sweeps, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (accept)
+        for (int i = 0; i < n_sweeps; i++) accept[i] = (double)counts[i] / (4.0 * n_hit * (double)c->geom.v);
+    return GLACX_OK;
+}
+
+int glacx_measure(glacx_ctx *c, int field, double *plaq, double *e_raw, double *q, double *q_t)
+{
+    CTX_CHECK(c);
+    if (!valid_field(field)) return fail(c, GLACX_E_ARG, "measure: bad field");
+    int rc = DISPATCH(c, run_measure, c, c->field[field], c->results);
+    if (rc) return rc;
+    std::vector<double> h((size_t)3 + c->dims[3]);
+    HIP_TRY(c, hipMemcpyAsync(h.data(), c->results, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (plaq) *plaq = h[0];
+    if (e_raw) *e_raw = h[1];
+    if (q) *q = h[2];
+    if (q_t) std::memcpy(q_t, h.data() + 3, sizeof(double) * c->dims[3]);
+    return GLACX_OK;
+}
+
+int glacx_plaquette(glacx_ctx *c, int field, double *plaq)
+{
+    CTX_CHECK(c);
+    if (!valid_field(field) || !plaq) return fail(c, GLACX_E_ARG, "plaquette: bad field or pointer");
+    int rc = DISPATCH(c, run_plaquette, c, c->field[field], c->results);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(plaq, c->results, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GLACX_OK;
+}
+
+int glacx_flow(glacx_ctx *c, int field, int n_steps, double eps, double *plaq, double *e_raw, double *q, double *q_t)
+{
+    CTX_CHECK(c);
+    if (!valid_field(field) || n_steps < 0) return fail(c, GLACX_E_ARG, "flow: bad field or step count");
+    const bool meas = plaq || e_raw || q || q_t;
+    const size_t row = (size_t)3 + c->dims[3];
+    int rc;
+    if (meas && (rc = ensure(c, &c->results, &c->results_len, row * (size_t)(n_steps + 1)))) return rc;
+    void *f = c->field[field];
+    if (meas && (rc = DISPATCH(c, run_measure, c, f, c->results))) return rc;
+    for (int i = 0; i < n_steps; i++) {
+        if ((rc = DISPATCH(c, run_flow_step, c, f, eps))) return rc;
+        // three stages leave the result in the work buffer: swap roles
+        std::swap(c->field[field], c->work);
+        f = c->field[field];
+        if (meas && (rc = DISPATCH(c, run_measure, c, f, c->results + row * (size_t)(i + 1)))) return rc;
+    }
+    if (meas) {
+        std::vector<double> h(row * (size_t)(n_steps + 1));
+        HIP_TRY(c, hipMemcpyAsync(h.data(), c->results, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        for (int i = 0; i <= n_steps; i++) {
+            const double *r = h.data() + row * i;
+            if (plaq) plaq[i] = r[0];
+            if (e_raw) e_raw[i] = r[1];
+            if (q) q[i] = r[2];
+            if (q_t) std::memcpy(q_t + (size_t)i * c->dims[3], r + 3, sizeof(double) * c->dims[3]);
+        }
+    } else {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    return GLACX_OK;
+}
+
+int glacx_action_derivative(glacx_ctx *c, int field, int mu, double *z_out)
+{
+    CTX_CHECK(c);
+    if (!valid_field(field) || mu < 0 || mu > 3 || !z_out)
+        return fail(c, GLACX_E_ARG, "action_derivative: bad field, mu or pointer");
+    int rc = ensure_stage(c);
+    if (rc) return rc;
+    DISPATCH(c, launch_action_derivative, c->field[field], c->stage, c->geom, mu, c->stream);
+    CHECK_LAUNCH(c);
+    HIP_TRY(c, hipMemcpyAsync(z_out, c->stage, (size_t)c->geom.v * 18 * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GLACX_OK;
+}
+
+int glacx_timing_enable(glacx_ctx *c, int on)
+{
+    CTX_CHECK(c);
+    c->timing = on != 0;
+    return GLACX_OK;
+}
+
+int glacx_timing_reset(glacx_ctx *c)
+{
+    CTX_CHECK(c);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->timed.clear();
+    c->ev_used = 0;
+    for (int k = 0; k < GLACX_K_COUNT; k++) {
+        c->t_launches[k] = 0;
+        c->t_ms[k] = 0.0;
+    }
+    return GLACX_OK;
+}
+
+int glacx_timing_read(glacx_ctx *c, int kernel, long long *launches, double *total_ms)
+{
+    CTX_CHECK(c);
+    if (kernel < 0 || kernel >= GLACX_K_COUNT) return fail(c, GLACX_E_ARG, "timing_read: bad kernel id");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (auto &t : c->timed) {
+        float ms = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, t.second.first, t.second.second));
+        c->t_launches[t.first]++;
+        c->t_ms[t.first] += ms;
+    }
+    c->timed.clear();
+    c->ev_used = 0;
+    if (launches) *launches = c->t_launches[kernel];
+    if (total_ms) *total_ms = c->t_ms[kernel];
+    return GLACX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// k_flow.hip -- one fused stage of Luscher's RK3 Wilson flow.
+//
+// GLAC's Flow::flowField (GLAC/flow/flow.cpp:43-73) runs each stage as four
+// whole-lattice getActionDerivative passes (GLAC/actions/wilsongaugeaction.cpp:
+// 55-86), a scale/combine pass, and four matrixExp passes (flow.cpp:95-115),
+// each materialising lattice temporaries.  Here a stage is one kernel per
+// link: staple -> Omega = U A -> Z = -P_TA(Omega) -> RK3 combination with the
+// accumulator -> exp (FP64 Cayley-Hamilton coefficients) -> X U, reading the
+// field from `win` and writing `wout` (ping-pong, so every Z of a stage sees
+// the same field, as GLAC's separate loops do).
+//
+//   stage 0:  acc <- Z0                       W1 = exp(eps/4 Z0) W0
+//   stage 1:  Y = 8/9 eps Z1 - 17/36 eps acc;  acc <- Y;  W2 = exp(Y) W1
+//   stage 2:  Y = 3/4 eps Z2 - acc;                      W3 = exp(Y) W2
+//
+// The accumulator holds the 8 real parameters of an su(3) element per link
+// (8 planes of T per direction).
+//
+// Launch: 256-thread blocks = 64 sites x 4 directions; wavefront w handles
+// mu = w for the block's 64 consecutive sites (coalesced plane accesses), so
+// the four links of a site -- which share most staple links -- are processed
+// on the same CU at the same time.
+#include "expsu3.hpp"
+#include "kernels.hpp"
+
+namespace glacx {
+
+template <class T, int STAGE>
+__global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<T>::type *__restrict__ win,
+                                                            typename vec2<T>::type *__restrict__ wout,
+                                                            T *__restrict__ acc, Geom g, T cz, T ca)
+{
+    const int mu = threadIdx.x >> 6;
+    const int idx = blockIdx.x * kFlowSites + (threadIdx.x & 63);
+    if (idx >= g.v) return;
+    const Coord x = decode(g, idx);
+    const M3<T> a = staple<T>(win, g, x, mu);
+    const M3<T> u = load_link<T>(win, g.v, mu, idx);
+    const su3alg<T> z = force_from_omega(mul(u, a));
+    T *ap = acc + (size_t)mu * 8 * g.v + idx;
+    su3alg<T> y;
+    if (STAGE == 0) {
+        ap[0] = z.a0;
+        ap[(size_t)g.v] = z.a1;
+#pragma unroll
+        for (int k = 0; k < 6; k++) ap[(size_t)(k + 2) * g.v] = z.p[k];
+        y.a0 = cz * z.a0;
+        y.a1 = cz * z.a1;
+#pragma unroll
+        for (int k = 0; k < 6; k++) y.p[k] = cz * z.p[k];
+    } else {
+        y.a0 = cz * z.a0 - ca * ap[0];
+        y.a1 = cz * z.a1 - ca * ap[(size_t)g.v];
+#pragma unroll
+        for (int k = 0; k < 6; k++) y.p[k] = cz * z.p[k] - ca * ap[(size_t)(k + 2) * g.v];
+        if (STAGE == 1) {
+            ap[0] = y.a0;
+            ap[(size_t)g.v] = y.a1;
+#pragma unroll
+            for (int k = 0; k < 6; k++) ap[(size_t)(k + 2) * g.v] = y.p[k];
+        }
+    }
+    store_link<T>(wout, g.v, mu, idx, mul(exp_su3(y), u));
+}
+
+template <class T>
+void launch_flow_stage(const void *win, void *wout, void *acc, const Geom &g, int stage, double eps, hipStream_t s)
+{
+    using V2 = typename vec2<T>::type;
+    const unsigned blocks = (unsigned)((g.v + kFlowSites - 1) / kFlowSites);
+    const V2 *in = (const V2 *)win;
+    V2 *out = (V2 *)wout;
+    T *a = (T *)acc;
+    switch (stage) {
+    case 0:
+        flow_stage_kernel<T, 0><<<blocks, kBlock, 0, s>>>(in, out, a, g, T(0.25 * eps), T(0));
+        break;
+    case 1:
+        flow_stage_kernel<T, 1><<<blocks, kBlock, 0, s>>>(in, out, a, g, T(8.0 / 9.0 * eps), T(17.0 / 36.0 * eps));
+        break;
+    default:
+        flow_stage_kernel<T, 2><<<blocks, kBlock, 0, s>>>(in, out, a, g, T(0.75 * eps), T(1));
+        break;
+    }
+}
+
+template void launch_flow_stage<float>(const void *, void *, void *, const Geom &, int, double, hipStream_t);
+template void launch_flow_stage<double>(const void *, void *, void *, const Geom &, int, double, hipStream_t);
+
+} // namespace glacx

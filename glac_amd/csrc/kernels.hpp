@@ -1,0 +1,79 @@
+// kernels.hpp -- host-side launchers of the gfx950 kernels (one translation
+// unit per kernel family) and the shared block reduction.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lattice.hpp"
+
+namespace glacx {
+
+constexpr int kBlock = 256;  // 4 wavefronts of 64
+
+// Flow-stage launch shape: a block covers 64 consecutive sites, wavefront w
+// handling direction mu = w, so the four links of a site share a CU.
+constexpr int kFlowSites = 64;
+
+struct MetroParams {
+    double beta;
+    double eps;
+    int n_hit;
+    int reunit;
+    uint64_t seed;
+    uint64_t sweep;
+};
+
+// Sum `n` per-thread doubles over a 256-thread block; thread 0 gets the sums.
+// Wavefront shuffle, then one LDS exchange between the four wavefronts; the
+// order of additions is fixed, so results are reproducible run to run.
+template <int N> __device__ __forceinline__ void block_sum(double (&v)[N])
+{
+    __shared__ double lds[N][kBlock / 64];
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        double x = v[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        v[k] = x;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < N; k++) lds[k][wave] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; k++) v[k] = (lds[k][0] + lds[k][1]) + (lds[k][2] + lds[k][3]);
+    }
+}
+
+// ---- launchers (explicitly instantiated for float and double) ------------
+template <class T> void launch_pack(const double *stage, void *f, const Geom &g, int site_major, hipStream_t s);
+template <class T> void launch_unpack(const void *f, double *stage, const Geom &g, int site_major, hipStream_t s);
+template <class T> void launch_hot_start(void *f, const Geom &g, uint64_t seed, hipStream_t s);
+template <class T> void launch_cold_start(void *f, const Geom &g, hipStream_t s);
+template <class T>
+void launch_action_derivative(const void *f, double *stage, const Geom &g, int mu, hipStream_t s);
+
+// Clover measurement: per-block partial sums, then a one-block fixed-order
+// reduction into out[0..2+Nt] = {plaq, E_raw, Q, Q(t)...}.
+int measure_partials_count(const Geom &g);
+template <class T> void launch_measure(const void *f, const Geom &g, double *partials, hipStream_t s);
+void launch_measure_finish(const double *partials, const Geom &g, double *out, hipStream_t s);
+
+int plaquette_partials_count(const Geom &g);
+template <class T> void launch_plaquette(const void *f, const Geom &g, double *partials, hipStream_t s);
+void launch_plaquette_finish(const double *partials, int nparts, const Geom &g, double *out, hipStream_t s);
+
+// One RK3 stage (0, 1, 2): win -> wout, accumulator read/written in acc.
+template <class T>
+void launch_flow_stage(const void *win, void *wout, void *acc, const Geom &g, int stage, double eps, hipStream_t s);
+
+// One checkerboard sub-sweep (mu, parity), in place; accepted hits are added
+// to *count.
+template <class T>
+void launch_metropolis(void *f, const Geom &g, int mu, int parity, const MetroParams &p,
+                       unsigned long long *count, hipStream_t s);
+
+} // namespace glacx

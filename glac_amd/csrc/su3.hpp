@@ -1,0 +1,291 @@
+// su3.hpp -- register-resident 3x3 complex algebra for the gfx950 kernels.
+//
+// A link is held as 9 complex entries in registers, row-major.  All products
+// are written out so the compiler schedules them as independent FMAs; the
+// real type T is float (FP32 links) or double (FP64 links).
+//
+// Products follow GLAC's SU3 operators (GLAC/math/matrices/su3.h:184-221,
+// 307-337) and traceRealMultiplication (GLAC/math/functions.h:45-53).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace glacx {
+
+template <class T> struct cpx {
+    T r, i;
+};
+
+template <class T> struct M3 {
+    cpx<T> e[3][3];
+};
+
+template <class T> __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b)
+{
+    return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r};
+}
+
+// a * conj(b)
+template <class T> __device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b)
+{
+    return {a.r * b.r + a.i * b.i, a.i * b.r - a.r * b.i};
+}
+
+// conj(a) * conj(b)
+template <class T> __device__ __forceinline__ cpx<T> ccmulc(cpx<T> a, cpx<T> b)
+{
+    return {a.r * b.r - a.i * b.i, -(a.r * b.i + a.i * b.r)};
+}
+
+// conj(a) * b
+template <class T> __device__ __forceinline__ cpx<T> cmulc_l(cpx<T> a, cpx<T> b)
+{
+    return {a.r * b.r + a.i * b.i, a.r * b.i - a.i * b.r};
+}
+
+template <class T> __device__ __forceinline__ M3<T> zero3()
+{
+    M3<T> m;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) m.e[i][j] = {T(0), T(0)};
+    return m;
+}
+
+template <class T> __device__ __forceinline__ M3<T> unit3()
+{
+    M3<T> m = zero3<T>();
+    m.e[0][0].r = m.e[1][1].r = m.e[2][2].r = T(1);
+    return m;
+}
+
+// C = A B
+template <class T> __device__ __forceinline__ M3<T> mul(const M3<T> &a, const M3<T> &b)
+{
+    M3<T> c;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            T re = a.e[i][0].r * b.e[0][j].r - a.e[i][0].i * b.e[0][j].i;
+            T im = a.e[i][0].r * b.e[0][j].i + a.e[i][0].i * b.e[0][j].r;
+            re += a.e[i][1].r * b.e[1][j].r - a.e[i][1].i * b.e[1][j].i;
+            im += a.e[i][1].r * b.e[1][j].i + a.e[i][1].i * b.e[1][j].r;
+            re += a.e[i][2].r * b.e[2][j].r - a.e[i][2].i * b.e[2][j].i;
+            im += a.e[i][2].r * b.e[2][j].i + a.e[i][2].i * b.e[2][j].r;
+            c.e[i][j] = {re, im};
+        }
+    return c;
+}
+
+// C = A B^dagger
+template <class T> __device__ __forceinline__ M3<T> mul_ad(const M3<T> &a, const M3<T> &b)
+{
+    M3<T> c;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            // sum_k a_ik conj(b_jk)
+            T re = a.e[i][0].r * b.e[j][0].r + a.e[i][0].i * b.e[j][0].i;
+            T im = a.e[i][0].i * b.e[j][0].r - a.e[i][0].r * b.e[j][0].i;
+            re += a.e[i][1].r * b.e[j][1].r + a.e[i][1].i * b.e[j][1].i;
+            im += a.e[i][1].i * b.e[j][1].r - a.e[i][1].r * b.e[j][1].i;
+            re += a.e[i][2].r * b.e[j][2].r + a.e[i][2].i * b.e[j][2].i;
+            im += a.e[i][2].i * b.e[j][2].r - a.e[i][2].r * b.e[j][2].i;
+            c.e[i][j] = {re, im};
+        }
+    return c;
+}
+
+// C = A^dagger B
+template <class T> __device__ __forceinline__ M3<T> mul_da(const M3<T> &a, const M3<T> &b)
+{
+    M3<T> c;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            // sum_k conj(a_ki) b_kj
+            T re = a.e[0][i].r * b.e[0][j].r + a.e[0][i].i * b.e[0][j].i;
+            T im = a.e[0][i].r * b.e[0][j].i - a.e[0][i].i * b.e[0][j].r;
+            re += a.e[1][i].r * b.e[1][j].r + a.e[1][i].i * b.e[1][j].i;
+            im += a.e[1][i].r * b.e[1][j].i - a.e[1][i].i * b.e[1][j].r;
+            re += a.e[2][i].r * b.e[2][j].r + a.e[2][i].i * b.e[2][j].i;
+            im += a.e[2][i].r * b.e[2][j].i - a.e[2][i].i * b.e[2][j].r;
+            c.e[i][j] = {re, im};
+        }
+    return c;
+}
+
+// C = A^dagger B^dagger = (B A)^dagger
+template <class T> __device__ __forceinline__ M3<T> mul_dd(const M3<T> &a, const M3<T> &b)
+{
+    M3<T> c;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            // sum_k conj(a_ki) conj(b_jk)
+            T re = a.e[0][i].r * b.e[j][0].r - a.e[0][i].i * b.e[j][0].i;
+            T im = -(a.e[0][i].r * b.e[j][0].i + a.e[0][i].i * b.e[j][0].r);
+            re += a.e[1][i].r * b.e[j][1].r - a.e[1][i].i * b.e[j][1].i;
+            im -= a.e[1][i].r * b.e[j][1].i + a.e[1][i].i * b.e[j][1].r;
+            re += a.e[2][i].r * b.e[j][2].r - a.e[2][i].i * b.e[j][2].i;
+            im -= a.e[2][i].r * b.e[j][2].i + a.e[2][i].i * b.e[j][2].r;
+            c.e[i][j] = {re, im};
+        }
+    return c;
+}
+
+template <class T> __device__ __forceinline__ void add_to(M3<T> &c, const M3<T> &a)
+{
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            c.e[i][j].r += a.e[i][j].r;
+            c.e[i][j].i += a.e[i][j].i;
+        }
+}
+
+template <class T> __device__ __forceinline__ void sub_from(M3<T> &c, const M3<T> &a)
+{
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            c.e[i][j].r -= a.e[i][j].r;
+            c.e[i][j].i -= a.e[i][j].i;
+        }
+}
+
+// Re tr(A B)
+template <class T> __device__ __forceinline__ T retrace_mul(const M3<T> &a, const M3<T> &b)
+{
+    T s = T(0);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) s += a.e[i][k].r * b.e[k][i].r - a.e[i][k].i * b.e[k][i].i;
+    return s;
+}
+
+template <class T> __device__ __forceinline__ T retrace(const M3<T> &a)
+{
+    return a.e[0][0].r + a.e[1][1].r + a.e[2][2].r;
+}
+
+template <class To, class From> __device__ __forceinline__ M3<To> convert(const M3<From> &a)
+{
+    M3<To> c;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) c.e[i][j] = {To(a.e[i][j].r), To(a.e[i][j].i)};
+    return c;
+}
+
+// Traceless antihermitian part of C - C^dagger, i.e. C <- (C - C^dag) - 1/3 tr(C - C^dag).
+// (GLAC/observables/mastersamplertopcxyz.cpp:331-340)
+template <class T> __device__ __forceinline__ M3<T> traceless_antiherm2(const M3<T> &c)
+{
+    M3<T> r;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            r.e[i][j].r = c.e[i][j].r - c.e[j][i].r;
+            r.e[i][j].i = c.e[i][j].i + c.e[j][i].i;
+        }
+    T third = (r.e[0][0].i + r.e[1][1].i + r.e[2][2].i) * T(1.0 / 3.0);
+    r.e[0][0].i -= third;
+    r.e[1][1].i -= third;
+    r.e[2][2].i -= third;
+    return r;
+}
+
+// Compact su(3) element: traceless antihermitian Y with
+//   Y_00 = i a0, Y_11 = i a1, Y_22 = -i (a0 + a1),
+//   Y_01 = (p[0], p[1]), Y_02 = (p[2], p[3]), Y_12 = (p[4], p[5]),
+//   Y_kj = -conj(Y_jk).
+template <class T> struct su3alg {
+    T a0, a1, p[6];
+};
+
+// Flow force Z = 1/2 [ (O^dag - O) - 1/3 Im tr(O^dag - O) i 1 ],  O = U A
+// (GLAC/actions/wilsongaugeaction.cpp:77-85).
+template <class T> __device__ __forceinline__ su3alg<T> force_from_omega(const M3<T> &o)
+{
+    su3alg<T> z;
+    T tr = (o.e[0][0].i + o.e[1][1].i + o.e[2][2].i) * T(1.0 / 3.0);
+    z.a0 = tr - o.e[0][0].i;
+    z.a1 = tr - o.e[1][1].i;
+    // Z_jk = 1/2 (conj(O_kj) - O_jk)
+    z.p[0] = T(0.5) * (o.e[1][0].r - o.e[0][1].r);
+    z.p[1] = T(-0.5) * (o.e[1][0].i + o.e[0][1].i);
+    z.p[2] = T(0.5) * (o.e[2][0].r - o.e[0][2].r);
+    z.p[3] = T(-0.5) * (o.e[2][0].i + o.e[0][2].i);
+    z.p[4] = T(0.5) * (o.e[2][1].r - o.e[1][2].r);
+    z.p[5] = T(-0.5) * (o.e[2][1].i + o.e[1][2].i);
+    return z;
+}
+
+template <class T> __device__ __forceinline__ M3<T> expand(const su3alg<T> &y)
+{
+    M3<T> m;
+    m.e[0][0] = {T(0), y.a0};
+    m.e[1][1] = {T(0), y.a1};
+    m.e[2][2] = {T(0), -(y.a0 + y.a1)};
+    m.e[0][1] = {y.p[0], y.p[1]};
+    m.e[0][2] = {y.p[2], y.p[3]};
+    m.e[1][2] = {y.p[4], y.p[5]};
+    m.e[1][0] = {-y.p[0], y.p[1]};
+    m.e[2][0] = {-y.p[2], y.p[3]};
+    m.e[2][1] = {-y.p[4], y.p[5]};
+    return m;
+}
+
+// Row Gram-Schmidt in FP64; row 2 = conj(row0 x row1) so det = +1.
+__device__ __forceinline__ void reunitarize(M3<double> &m)
+{
+    double n0 = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) n0 += m.e[0][j].r * m.e[0][j].r + m.e[0][j].i * m.e[0][j].i;
+    n0 = 1.0 / sqrt(n0);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        m.e[0][j].r *= n0;
+        m.e[0][j].i *= n0;
+    }
+    double pr = 0.0, pi = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        pr += m.e[0][j].r * m.e[1][j].r + m.e[0][j].i * m.e[1][j].i;
+        pi += m.e[0][j].r * m.e[1][j].i - m.e[0][j].i * m.e[1][j].r;
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        double ar = m.e[0][j].r, ai = m.e[0][j].i;
+        m.e[1][j].r -= pr * ar - pi * ai;
+        m.e[1][j].i -= pr * ai + pi * ar;
+    }
+    double n1 = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) n1 += m.e[1][j].r * m.e[1][j].r + m.e[1][j].i * m.e[1][j].i;
+    n1 = 1.0 / sqrt(n1);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        m.e[1][j].r *= n1;
+        m.e[1][j].i *= n1;
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int k = (j + 1) % 3, l = (j + 2) % 3;
+        cpx<double> x = cmul(m.e[0][k], m.e[1][l]);
+        cpx<double> y = cmul(m.e[0][l], m.e[1][k]);
+        m.e[2][j] = {x.r - y.r, -(x.i - y.i)};
+    }
+}
+
+} // namespace glacx

@@ -1,0 +1,141 @@
+/*
+ * glacx.h -- C ABI of libglacx.so, the MI355X (gfx950) implementation of
+ * GLAC's gauge-generation and gradient-flow hot path.
+ *
+ * Plain C: pointers, sizes and integer status codes; no C++ or torch types.
+ * Every call returns 0 on success and a negative GLACX_E_* code on failure;
+ * glacx_last_error() then returns the message.  A context owns one HIP stream
+ * (default priority) and all device memory; calls on one context must come
+ * from one host thread at a time.  Results written through host pointers are
+ * complete when the call returns.
+ *
+ * Host-side arrays use GLAC's layout: a link is 18 doubles, row-major 3x3
+ * complex with (re, im) interleaved (GLAC/math/matrices/su3.h:22,64); a gauge
+ * field is four arrays, one per mu, of V links in site order
+ * x + Nx(y + Ny(z + Nz t)) (GLAC/parallelization/index.h:32-35,
+ * GLAC/system.cpp:158-161).
+ *
+ * A context holds two gauge fields, GLACX_FIELD_MAIN (GLAC's m_lattice, the
+ * Markov chain) and GLACX_FIELD_FLOW (GLAC's m_flowLattice), resident on the
+ * device between calls (GLAC/system.cpp:138-188).
+ */
+#ifndef GLACX_H
+#define GLACX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct glacx_ctx glacx_ctx;
+
+enum glacx_precision {
+    GLACX_FP64 = 0,           /* links, staples, exponential and sums in FP64 */
+    GLACX_FP32_LINKS = 1      /* FP32 links/staples/flow; FP64 exp coefficients,
+                                 reunitarisation and lattice-wide sums */
+};
+
+enum glacx_field { GLACX_FIELD_MAIN = 0, GLACX_FIELD_FLOW = 1 };
+
+enum glacx_status {
+    GLACX_OK = 0,
+    GLACX_E_ARG = -1,         /* bad argument (sizes, null pointers, odd extents) */
+    GLACX_E_HIP = -2,         /* HIP runtime error */
+    GLACX_E_IO = -3,          /* file missing / short read / write failure */
+    GLACX_E_STATE = -4        /* call not valid in the current state */
+};
+
+/* Timed kernel families for glacx_timing_read. */
+enum glacx_kernel {
+    GLACX_K_FLOW_STAGE = 0,
+    GLACX_K_MEASURE = 1,
+    GLACX_K_METROPOLIS = 2,
+    GLACX_K_PLAQUETTE = 3,
+    GLACX_K_COUNT = 4
+};
+
+/* Create a context for an Nx x Ny x Nz x Nt periodic lattice (all extents
+ * even and >= 2) on HIP device `device`.  Replaces GLAC's lattice allocation
+ * in System::subLatticeSetup (GLAC/system.cpp:148-177) and Flow's
+ * temporaries (GLAC/flow/flow.cpp:14-25). */
+int glacx_create(const int dims[4], int precision, int device, glacx_ctx **out);
+int glacx_destroy(glacx_ctx *ctx);
+const char *glacx_last_error(const glacx_ctx *ctx);
+/* Device bytes held by the context. */
+int glacx_device_bytes(const glacx_ctx *ctx, size_t *bytes);
+
+/* Host <-> device field transfer in GLAC's layout: links[mu] points at V*18
+ * doubles (a Lattice<SU3>'s site vector).  Replaces the host-side lattice
+ * ownership of GLAC/system.cpp:158-161. */
+int glacx_upload_field(glacx_ctx *ctx, int field, const double *const links[4]);
+int glacx_download_field(glacx_ctx *ctx, int field, double *const links[4]);
+/* copyToFlowLattice (GLAC/system.cpp:179-188): dst <- src on the device. */
+int glacx_copy_field(glacx_ctx *ctx, int dst, int src);
+
+/* GLAC .bin configuration (GLAC/io/fieldio.cpp:61-122, 195-248): sites in
+ * x-fastest order, 4 links per site (mu = 0..3), 18 native-endian doubles
+ * per link, no header; 576 V bytes. */
+int glacx_load_bin(glacx_ctx *ctx, int field, const char *path);
+int glacx_save_bin(glacx_ctx *ctx, int field, const char *path);
+
+/* Starts (GLAC/system.cpp:200-224).  Hot start draws Haar-random SU(3) links
+ * from the counter-based stream (seed, site, mu); cold start sets unity. */
+int glacx_cold_start(glacx_ctx *ctx, int field);
+int glacx_hot_start(glacx_ctx *ctx, int field, uint64_t seed);
+
+/* n_sweeps Metropolis sweeps on GLACX_FIELD_MAIN (System::update,
+ * GLAC/system.cpp:333-361, with WilsonGaugeAction::computeStaple /
+ * getDeltaAction, GLAC/actions/wilsongaugeaction.cpp:24-53, and GLAC's RST
+ * proposal, GLAC/math/matrices/su3matrixgenerator.h:170-321).  A sweep is 8
+ * checkerboard sub-sweeps (mu = 0..3, parity even then odd); each link gets
+ * n_hit proposals X U accepted iff exp(-dS) > u.  Links are reunitarised
+ * (FP64) after their hits when reunit_every > 0 and the global sweep index
+ * (sweep_offset + i) is a multiple of reunit_every.  Sweep i draws from the
+ * streams of (seed, sweep_offset + i).  accept[i] (may be NULL) receives the
+ * acceptance rate accepted / (4 n_hit V) of sweep i (GLAC/system.cpp:359);
+ * plaq[i] (may be NULL) the plaquette after sweep i. */
+int glacx_metropolis(glacx_ctx *ctx, int n_sweeps, double beta, double eps, int n_hit,
+                     int reunit_every, uint64_t seed, uint64_t sweep_offset,
+                     double *accept, double *plaq);
+
+/* Clover observables of one field (MasterSamplerTopcXYZ::calculate,
+ * GLAC/observables/mastersamplertopcxyz.cpp:236-384): plaquette
+ * (normalised to 1 at unity), E_raw (physical E = -E_raw / 64), Q, and
+ * q_t[Nt], the charge summed over each time slice.  Any pointer may be NULL. */
+int glacx_measure(glacx_ctx *ctx, int field, double *plaq, double *e_raw, double *q, double *q_t);
+
+/* Plaquette only (Plaquette::calculate, GLAC/observables/plaquette.cpp:26-40). */
+int glacx_plaquette(glacx_ctx *ctx, int field, double *plaq);
+
+/* n_steps RK3 Wilson-flow steps of size eps on `field`, in place
+ * (Flow::flowField, GLAC/flow/flow.cpp:43-73, with the SU3Exp exponential,
+ * GLAC/math/exponentiation/su3exp.cpp:16-79).  When the output pointers are
+ * non-NULL the clover observables are measured before the first step and
+ * after every step, as System::flowConfiguration does
+ * (GLAC/system.cpp:471-517): plaq, e_raw and q hold n_steps + 1 values and
+ * q_t holds (n_steps + 1) * Nt values, row i = flow time i * eps. */
+int glacx_flow(glacx_ctx *ctx, int field, int n_steps, double eps,
+               double *plaq, double *e_raw, double *q, double *q_t);
+
+/* Whole-lattice flow force for one direction in GLAC's layout, V*18 doubles
+ * (WilsonGaugeAction::getActionDerivative,
+ * GLAC/actions/wilsongaugeaction.cpp:55-86). */
+int glacx_action_derivative(glacx_ctx *ctx, int field, int mu, double *z_out);
+
+/* Block until the context's stream is idle. */
+int glacx_synchronize(glacx_ctx *ctx);
+
+/* Per-kernel device time with HIP events on the context's stream.  While
+ * enabled, every launch of a timed kernel family is bracketed by events;
+ * glacx_timing_read synchronises and returns the launch count and the summed
+ * device milliseconds since the last reset. */
+int glacx_timing_enable(glacx_ctx *ctx, int on);
+int glacx_timing_reset(glacx_ctx *ctx);
+int glacx_timing_read(glacx_ctx *ctx, int kernel, long long *launches, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GLACX_H */
