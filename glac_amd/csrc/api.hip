@@ -212,17 +212,16 @@ int glacx_create(const int dims[4], int precision, int device, glacx_ctx **out)
     *out = nullptr;
     long v = 1;
     for (int k = 0; k < 4; k++) {
-        if (dims[k] < 2 || (dims[k] & 1)) return GLACX_E_ARG;
+        if (dims[k] < 2 || (dims[k] & 1) || dims[k] > 512) return GLACX_E_ARG;
         v *= dims[k];
     }
-    if (v > (1L << 30)) return GLACX_E_ARG;
+    // the 9 planes of one direction are addressed by one buffer resource
+    if ((unsigned long long)v * 9ull * 2ull * (precision == GLACX_FP32_LINKS ? 4ull : 8ull) >= (1ull << 32))
+        return GLACX_E_ARG;
     if (precision != GLACX_FP64 && precision != GLACX_FP32_LINKS) return GLACX_E_ARG;
     glacx_ctx *c = new glacx_ctx;
-    for (int k = 0; k < 4; k++) c->dims[k] = c->geom.n[k] = dims[k];
-    c->geom.nxh = dims[0] / 2;
-    c->geom.v = (int)v;
-    c->geom.vh = (int)(v / 2);
-    c->geom.vs_h = dims[0] * dims[1] * dims[2] / 2;
+    for (int k = 0; k < 4; k++) c->dims[k] = dims[k];
+    c->geom = make_geom(dims);
     c->precision = precision;
     c->real_size = precision == GLACX_FP32_LINKS ? 4 : 8;
     c->device = device;
@@ -246,13 +245,13 @@ int glacx_create(const int dims[4], int precision, int device, glacx_ctx **out)
 int glacx_destroy(glacx_ctx *c)
 {
     if (!c) return GLACX_E_ARG;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (void *p : {c->field[0], c->field[1], c->work, c->acc, (void *)c->partials, (void *)c->results,
                     (void *)c->counts, (void *)c->stage})
-        if (p) hipFree(p);
-    for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (p) (void)hipFree(p);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return GLACX_OK;
 }

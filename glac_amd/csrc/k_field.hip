@@ -68,7 +68,7 @@ __global__ __launch_bounds__(kBlock) void hot_start_kernel(typename vec2<T>::typ
         m.e[k / 3][k % 3] = {r * c, r * s};
     }
     reunitarize(m);
-    store_link<T>(f, g.v, mu, idx, convert<T>(m));
+    store_link_any<T>(f, g.v, mu, idx, convert<T>(m));
 }
 
 template <class T>
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(kBlock) void cold_start_kernel(typename vec2<T>::ty
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= 4 * g.v) return;
     const int mu = t / g.v, idx = t - mu * g.v;
-    store_link<T>(f, g.v, mu, idx, unit3<T>());
+    store_link_any<T>(f, g.v, mu, idx, unit3<T>());
 }
 
 template <class T>

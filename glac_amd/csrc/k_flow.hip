@@ -30,8 +30,8 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
                                                             typename vec2<T>::type *__restrict__ wout,
                                                             T *__restrict__ acc, Geom g, T cz, T ca)
 {
-    const int mu = threadIdx.x >> 6;
-    const int idx = blockIdx.x * kFlowSites + (threadIdx.x & 63);
+    const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // one direction per wavefront
+    const int idx = xcd_block(blockIdx.x, gridDim.x) * kFlowSites + (threadIdx.x & 63);
     if (idx >= g.v) return;
     const Coord x = decode(g, idx);
     const M3<T> a = staple<T>(win, g, x, mu);
@@ -63,6 +63,12 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
     store_link<T>(wout, g.v, mu, idx, mul(exp_su3(y), u));
 }
 
+// Dynamic LDS reserved per block; nonzero values cap the blocks resident per
+// CU (a tuning parameter of the launch, set at build time).
+#ifndef GLACX_FLOW_LDS_BYTES
+#define GLACX_FLOW_LDS_BYTES 0
+#endif
+
 template <class T>
 void launch_flow_stage(const void *win, void *wout, void *acc, const Geom &g, int stage, double eps, hipStream_t s)
 {
@@ -73,13 +79,13 @@ void launch_flow_stage(const void *win, void *wout, void *acc, const Geom &g, in
     T *a = (T *)acc;
     switch (stage) {
     case 0:
-        flow_stage_kernel<T, 0><<<blocks, kBlock, 0, s>>>(in, out, a, g, T(0.25 * eps), T(0));
+        flow_stage_kernel<T, 0><<<blocks, kBlock, GLACX_FLOW_LDS_BYTES, s>>>(in, out, a, g, T(0.25 * eps), T(0));
         break;
     case 1:
-        flow_stage_kernel<T, 1><<<blocks, kBlock, 0, s>>>(in, out, a, g, T(8.0 / 9.0 * eps), T(17.0 / 36.0 * eps));
+        flow_stage_kernel<T, 1><<<blocks, kBlock, GLACX_FLOW_LDS_BYTES, s>>>(in, out, a, g, T(8.0 / 9.0 * eps), T(17.0 / 36.0 * eps));
         break;
     default:
-        flow_stage_kernel<T, 2><<<blocks, kBlock, 0, s>>>(in, out, a, g, T(0.75 * eps), T(1));
+        flow_stage_kernel<T, 2><<<blocks, kBlock, GLACX_FLOW_LDS_BYTES, s>>>(in, out, a, g, T(0.75 * eps), T(1));
         break;
     }
 }

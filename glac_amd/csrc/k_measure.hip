@@ -10,10 +10,10 @@
 //   q -= Re tr(C1 C2).
 // Normalisations (:14-16): plaq / (18 V), E / V, Q / (256 pi^2).
 //
-// Each block covers one (parity, time slice) slab chunk, so its q partial
-// belongs to one time slice; per-site values are summed in FP64 and the
-// partials are combined in a fixed order by a single-block finishing kernel,
-// which makes the result independent of scheduling.
+// Each block covers 64 sites of one time slice, so its q partial belongs to
+// one slice; per-site values are summed in FP64, the block partials are
+// combined per time slice and then over slices in a fixed order, which makes
+// the result independent of scheduling.
 #include "kernels.hpp"
 
 namespace glacx {
@@ -26,56 +26,61 @@ __device__ __forceinline__ M3<T> clover(const typename vec2<T>::type *__restrict
     const Coord xmm = step(g, x, mu, -1), xmn = step(g, x, nu, -1);
     const Coord xpmmn = step(g, xpm, nu, -1), xmmmn = step(g, xmm, nu, -1), xmmpn = step(g, xmm, nu, +1);
     const int ix = index_of(g, x), imn = index_of(g, xmn), imm = index_of(g, xmm);
-    const M3<T> umu = load_link<T>(f, g.v, mu, ix);
-    const M3<T> unu = load_link<T>(f, g.v, nu, ix);
+    const DirView<T> vmu = dir_view<T>(f, g.v, mu), vnu = dir_view<T>(f, g.v, nu);
+    const M3<T> umu = load_link<T>(vmu, ix);
+    const M3<T> unu = load_link<T>(vnu, ix);
     // L1 = U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
-    M3<T> c = mul_ad(mul_ad(mul(umu, load_link<T>(f, g.v, nu, index_of(g, xpm))),
-                            load_link<T>(f, g.v, mu, index_of(g, xpn))),
-                     unu);
+    M3<T> c = mul_ad(mul_ad(mul(umu, load_link<T>(vnu, index_of(g, xpm))), load_link<T>(vmu, index_of(g, xpn))), unu);
     plaq = retrace(c);
-    // L2 = U_mu(x) U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu)
-    const M3<T> unu_mn = load_link<T>(f, g.v, nu, imn);
-    sub_from(c, mul(mul_ad(mul_ad(umu, load_link<T>(f, g.v, nu, index_of(g, xpmmn))),
-                           load_link<T>(f, g.v, mu, imn)),
-                    unu_mn));
-    // L3 = U_mu(x-mu)^dag U_nu(x-mu-nu)^dag U_mu(x-mu-nu) U_nu(x-nu)
-    const M3<T> umu_mm = load_link<T>(f, g.v, mu, imm);
+    // - L2 = -U_mu(x) U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu)
+    const M3<T> unu_mn = load_link<T>(vnu, imn);
+    {
+        M3<T> m = mul_ad(mul_ad(umu, load_link<T>(vnu, index_of(g, xpmmn))), load_link<T>(vmu, imn));
+        negate(m);
+        mm<false, false, true>(c, m, unu_mn);
+    }
+    // + L3 = U_mu(x-mu)^dag U_nu(x-mu-nu)^dag U_mu(x-mu-nu) U_nu(x-nu)
+    const M3<T> umu_mm = load_link<T>(vmu, imm);
     const int immmn = index_of(g, xmmmn);
-    add_to(c, mul(mul(mul_dd(umu_mm, load_link<T>(f, g.v, nu, immmn)), load_link<T>(f, g.v, mu, immmn)), unu_mn));
-    // L4 = U_mu(x-mu)^dag U_nu(x-mu) U_mu(x-mu+nu) U_nu(x)^dag
-    sub_from(c, mul_ad(mul(mul_da(umu_mm, load_link<T>(f, g.v, nu, imm)),
-                           load_link<T>(f, g.v, mu, index_of(g, xmmpn))),
-                       unu));
+    mm<false, false, true>(c, mul(mul_dd(umu_mm, load_link<T>(vnu, immmn)), load_link<T>(vmu, immmn)), unu_mn);
+    // - L4 = -U_mu(x-mu)^dag U_nu(x-mu) U_mu(x-mu+nu) U_nu(x)^dag
+    {
+        M3<T> m = mul(mul_da(umu_mm, load_link<T>(vnu, imm)), load_link<T>(vmu, index_of(g, xmmpn)));
+        negate(m);
+        mm<false, true, true>(c, m, unu);
+    }
     return c;
 }
 
+// Launch shape: 192-thread blocks = 64 sites x 3 wavefronts; wavefront w
+// evaluates the clover pair nu = w + 1 (planes (0, nu) and (rho, sigma)) for
+// the block's 64 sites, so the three pairs of a site run side by side.
+// Block b covers chunk c = b / Nt of time slice t = b % Nt: consecutive
+// blocks walk through t at fixed spatial position, so the t +- 1 links a
+// clover needs were just loaded by the neighbouring blocks.
+constexpr int kMeasSites = 64;
+constexpr int kMeasWaves = 3;
+
 template <class T>
-__global__ __launch_bounds__(kBlock) void measure_kernel(const typename vec2<T>::type *__restrict__ f, Geom g, int cps,
-                                                         double *__restrict__ partials)
+__global__ __launch_bounds__(kMeasSites *kMeasWaves) void measure_kernel(const typename vec2<T>::type *__restrict__ f,
+                                                                         Geom g, int cps, double *__restrict__ partials)
 {
-    const int b = blockIdx.x;
-    const int slab = b / cps, chunk = b - slab * cps;  // slab = parity * Nt + t
-    const int p = slab / g.n[3], t = slab - p * g.n[3];
-    const int r = chunk * kBlock + threadIdx.x;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int chunk = b / g.n[3], t = b - chunk * g.n[3];
+    const int r = chunk * kMeasSites + (threadIdx.x & 63);
+    const int nu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) + 1;  // one clover pair per wavefront
     double acc[3] = {0.0, 0.0, 0.0};
-    if (r < g.vs_h) {
-        const Coord x = decode(g, p * g.vh + t * g.vs_h + r);
-        T plaq = T(0), en = T(0), q = T(0);
-#pragma unroll 1
-        for (int nu = 1; nu < 4; nu++) {
-            const int rho = nu % 3 + 1, sigma = rho % 3 + 1;
-            T p1, p2;
-            const M3<T> c1 = traceless_antiherm2(clover<T>(f, g, x, 0, nu, p1));
-            const M3<T> c2 = traceless_antiherm2(clover<T>(f, g, x, rho, sigma, p2));
-            plaq += p1 + p2;
-            q -= retrace_mul(c1, c2);
-            en += retrace_mul(c1, c1) + retrace_mul(c2, c2);
-        }
-        acc[0] = plaq;
-        acc[1] = en;
-        acc[2] = q;
+    if (r < g.vs) {
+        const Coord x = decode(g, slice_index(g, t, r));
+        const int rho = nu % 3 + 1, sigma = rho % 3 + 1;
+        T p1, p2;
+        const M3<T> c1 = traceless_antiherm2(clover<T>(f, g, x, 0, nu, p1));
+        const M3<T> c2 = traceless_antiherm2(clover<T>(f, g, x, rho, sigma, p2));
+        acc[0] = p1 + p2;
+        acc[1] = retrace_mul(c1, c1) + retrace_mul(c2, c2);
+        acc[2] = -retrace_mul(c1, c2);
     }
-    block_sum<3>(acc);
+    block_sum<3, kMeasWaves>(acc);
     if (threadIdx.x == 0) {
         partials[3 * b + 0] = acc[0];
         partials[3 * b + 1] = acc[1];
@@ -83,32 +88,43 @@ __global__ __launch_bounds__(kBlock) void measure_kernel(const typename vec2<T>:
     }
 }
 
-// One block: out = {plaq, E_raw, Q, Q(t = 0..Nt-1)}.
-__global__ __launch_bounds__(kBlock) void measure_finish_kernel(const double *__restrict__ partials, Geom g, int cps,
-                                                                double *__restrict__ out)
+// One block per time slice t: sums the partials of the chunks of slice t in
+// a fixed order into slice[3 t .. 3 t + 2].
+__global__ __launch_bounds__(kBlock) void measure_slice_kernel(const double *__restrict__ partials, Geom g, int cps,
+                                                               double *__restrict__ slice)
 {
-    const int nblk = 2 * g.n[3] * cps;
-    double acc[2] = {0.0, 0.0};
-    for (int b = threadIdx.x; b < nblk; b += kBlock) {
-        acc[0] += partials[3 * b + 0];
-        acc[1] += partials[3 * b + 1];
+    const int t = blockIdx.x;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int c = threadIdx.x; c < cps; c += kBlock) {
+        const double *q = partials + 3 * (c * g.n[3] + t);
+        acc[0] += q[0];
+        acc[1] += q[1];
+        acc[2] += q[2];
     }
-    block_sum<2>(acc);
-    const double qnorm = 1.0 / (256.0 * M_PI * M_PI);
-    for (int t = threadIdx.x; t < g.n[3]; t += kBlock) {
-        double q = 0.0;
-        for (int p = 0; p < 2; p++)
-            for (int c = 0; c < cps; c++) q += partials[3 * ((p * g.n[3] + t) * cps + c) + 2];
-        out[3 + t] = q * qnorm;
-    }
-    __syncthreads();
+    block_sum<3>(acc);
     if (threadIdx.x == 0) {
-        double q = 0.0;
-        for (int t = 0; t < g.n[3]; t++) q += out[3 + t];
-        out[0] = acc[0] / (18.0 * (double)g.v);
-        out[1] = acc[1] / (double)g.v;
-        out[2] = q;
+        slice[3 * t + 0] = acc[0];
+        slice[3 * t + 1] = acc[1];
+        slice[3 * t + 2] = acc[2];
     }
+}
+
+// out = {plaq, E_raw, Q, Q(t = 0..Nt-1)}.
+__global__ void measure_finish_kernel(const double *__restrict__ slice, Geom g, double *__restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    const double qnorm = 1.0 / (256.0 * M_PI * M_PI);
+    double pl = 0.0, en = 0.0, q = 0.0;
+    for (int t = 0; t < g.n[3]; t++) {
+        pl += slice[3 * t + 0];
+        en += slice[3 * t + 1];
+        const double qt = slice[3 * t + 2] * qnorm;
+        out[3 + t] = qt;
+        q += qt;
+    }
+    out[0] = pl / (18.0 * (double)g.v);
+    out[1] = en / (double)g.v;
+    out[2] = q;
 }
 
 // Plaquette only: sum over sites and the 6 planes of
@@ -153,20 +169,25 @@ __global__ __launch_bounds__(kBlock) void plaquette_finish_kernel(const double *
     if (threadIdx.x == 0) out[0] = acc[0] / (18.0 * (double)g.v);
 }
 
-static inline int chunks_per_slab(const Geom &g) { return (g.vs_h + kBlock - 1) / kBlock; }
+static inline int chunks_per_slab(const Geom &g) { return (g.vs + kMeasSites - 1) / kMeasSites; }
 
-int measure_partials_count(const Geom &g) { return 3 * 2 * g.n[3] * chunks_per_slab(g); }
+// block partials followed by the per-slice sums
+int measure_partials_count(const Geom &g) { return 3 * g.n[3] * chunks_per_slab(g) + 3 * g.n[3]; }
 int plaquette_partials_count(const Geom &g) { return (g.v + kBlock - 1) / kBlock; }
 
 template <class T> void launch_measure(const void *f, const Geom &g, double *partials, hipStream_t s)
 {
     const int cps = chunks_per_slab(g);
-    measure_kernel<T><<<2 * g.n[3] * cps, kBlock, 0, s>>>((const typename vec2<T>::type *)f, g, cps, partials);
+    measure_kernel<T><<<g.n[3] * cps, kMeasSites * kMeasWaves, 0, s>>>((const typename vec2<T>::type *)f, g, cps,
+                                                                             partials);
 }
 
-void launch_measure_finish(const double *partials, const Geom &g, double *out, hipStream_t s)
+void launch_measure_finish(double *partials, const Geom &g, double *out, hipStream_t s)
 {
-    measure_finish_kernel<<<1, kBlock, 0, s>>>(partials, g, chunks_per_slab(g), out);
+    const int cps = chunks_per_slab(g);
+    double *slice = partials + 3 * g.n[3] * cps;
+    measure_slice_kernel<<<g.n[3], kBlock, 0, s>>>(partials, g, cps, slice);
+    measure_finish_kernel<<<1, 64, 0, s>>>(slice, g, out);
 }
 
 template <class T> void launch_plaquette(const void *f, const Geom &g, double *partials, hipStream_t s)

@@ -30,13 +30,14 @@ __device__ __forceinline__ void su2_near_unity(const T r0, const T r1, const T r
                                                cpx<T> &a, cpx<T> &b)
 {
     T x0 = r0 < T(0) ? -x0abs : x0abs;
-    const T rn = sqrt(r1 * r1 + r2 * r2 + r3 * r3) / eps;
-    T x1 = r1 / rn, x2 = r2 / rn, x3 = r3 / rn;
-    const T nrm = sqrt(x0 * x0 + x1 * x1 + x2 * x2 + x3 * x3);
-    x0 /= nrm;
-    x1 /= nrm;
-    x2 /= nrm;
-    x3 /= nrm;
+    const T sc = eps * rsqrt(fma(r1, r1, fma(r2, r2, r3 * r3)));  // eps / |r|
+    T x1 = r1 * sc, x2 = r2 * sc, x3 = r3 * sc;
+    // |x| = 1 up to rounding (x0^2 = 1 - eps^2); renormalise as GLAC does
+    const T inv = rsqrt(fma(x0, x0, fma(x1, x1, fma(x2, x2, x3 * x3))));
+    x0 *= inv;
+    x1 *= inv;
+    x2 *= inv;
+    x3 *= inv;
     a = {x0, x3};
     b = {x2, x1};
 }
@@ -75,6 +76,11 @@ __device__ __forceinline__ M3<T> rst_product(cpx<T> ra, cpx<T> rb, cpx<T> sa, cp
     return x;
 }
 
+// exp(-dS): FP64 libm exp for FP64 links; the hardware base-2 exponential for
+// FP32 links, whose dS already carries FP32 rounding.
+__device__ __forceinline__ double accept_prob(double ds) { return exp(-ds); }
+__device__ __forceinline__ double accept_prob(float ds) { return (double)__expf(-ds); }
+
 template <class T> __device__ __forceinline__ M3<T> dagger(const M3<T> &m)
 {
     M3<T> d;
@@ -90,10 +96,10 @@ __global__ __launch_bounds__(kBlock) void metropolis_kernel(typename vec2<T>::ty
                                                             int parity, MetroParams prm,
                                                             unsigned long long *__restrict__ count)
 {
-    const int h = blockIdx.x * kBlock + threadIdx.x;
+    const int h = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     double nacc[1] = {0.0};
     if (h < g.vh) {
-        const int idx = parity * g.vh + h;
+        const int idx = parity_index(g, parity, h);
         const Coord x = decode(g, idx);
         const M3<T> a = staple<T>(f, g, x, mu);
         M3<T> u = load_link<T>(f, g.v, mu, idx);
@@ -122,8 +128,8 @@ __global__ __launch_bounds__(kBlock) void metropolis_kernel(typename vec2<T>::ty
             const T coin = rs.centered<T>();
             M3<T> xm = rst_product(ra, rb, sa, sb, ta, tb);
             if (coin < T(0)) xm = dagger(xm);
-            const double ds = -bf * (double)(retrace_mul(xm, w) - trw);
-            if (exp(-ds) > rs.open()) {
+            const T ds = T(-bf) * (retrace_mul(xm, w) - trw);
+            if (accept_prob(ds) > rs.open()) {
                 u = mul(xm, u);
                 w = mul(xm, w);
                 trw = retrace(w);

@@ -23,12 +23,12 @@ struct MetroParams {
     uint64_t sweep;
 };
 
-// Sum `n` per-thread doubles over a 256-thread block; thread 0 gets the sums.
-// Wavefront shuffle, then one LDS exchange between the four wavefronts; the
-// order of additions is fixed, so results are reproducible run to run.
-template <int N> __device__ __forceinline__ void block_sum(double (&v)[N])
+// Sum `N` per-thread doubles over a block of WAVES wavefronts; thread 0 gets
+// the sums.  Wavefront shuffle, then one LDS exchange between the wavefronts;
+// the order of additions is fixed, so results are reproducible run to run.
+template <int N, int WAVES = kBlock / 64> __device__ __forceinline__ void block_sum(double (&v)[N])
 {
-    __shared__ double lds[N][kBlock / 64];
+    __shared__ double lds[N][WAVES];
 #pragma unroll
     for (int k = 0; k < N; k++) {
         double x = v[k];
@@ -44,7 +44,12 @@ template <int N> __device__ __forceinline__ void block_sum(double (&v)[N])
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < N; k++) v[k] = (lds[k][0] + lds[k][1]) + (lds[k][2] + lds[k][3]);
+        for (int k = 0; k < N; k++) {
+            double s = lds[k][0];
+#pragma unroll
+            for (int w = 1; w < WAVES; w++) s += lds[k][w];
+            v[k] = s;
+        }
     }
 }
 
@@ -60,7 +65,7 @@ void launch_action_derivative(const void *f, double *stage, const Geom &g, int m
 // reduction into out[0..2+Nt] = {plaq, E_raw, Q, Q(t)...}.
 int measure_partials_count(const Geom &g);
 template <class T> void launch_measure(const void *f, const Geom &g, double *partials, hipStream_t s);
-void launch_measure_finish(const double *partials, const Geom &g, double *out, hipStream_t s);
+void launch_measure_finish(double *partials, const Geom &g, double *out, hipStream_t s);
 
 int plaquette_partials_count(const Geom &g);
 template <class T> void launch_plaquette(const void *f, const Geom &g, double *partials, hipStream_t s);

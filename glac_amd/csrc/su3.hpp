@@ -59,22 +59,45 @@ template <class T> __device__ __forceinline__ M3<T> unit3()
     return m;
 }
 
-// C = A B
-template <class T> __device__ __forceinline__ M3<T> mul(const M3<T> &a, const M3<T> &b)
+// Generic 3x3 complex product written as FMA chains (one multiply and five
+// fused multiply-adds per real entry, or six FMAs when accumulating).
+// Element (i, k) of the left operand is a[i][k], or conj(a[k][i]) when DA is
+// set; element (k, j) of the right operand is b[k][j], or conj(b[j][k]) when
+// DB is set.  ACC adds the product to c instead of overwriting it.
+template <bool DA, bool DB, bool ACC, class T>
+__device__ __forceinline__ void mm(M3<T> &c, const M3<T> &a, const M3<T> &b)
 {
-    M3<T> c;
+    constexpr T sa = DA ? T(-1) : T(1);  // sign of the imaginary part of the left element
+    constexpr T sb = DB ? T(-1) : T(1);
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            T re = a.e[i][0].r * b.e[0][j].r - a.e[i][0].i * b.e[0][j].i;
-            T im = a.e[i][0].r * b.e[0][j].i + a.e[i][0].i * b.e[0][j].r;
-            re += a.e[i][1].r * b.e[1][j].r - a.e[i][1].i * b.e[1][j].i;
-            im += a.e[i][1].r * b.e[1][j].i + a.e[i][1].i * b.e[1][j].r;
-            re += a.e[i][2].r * b.e[2][j].r - a.e[i][2].i * b.e[2][j].i;
-            im += a.e[i][2].r * b.e[2][j].i + a.e[i][2].i * b.e[2][j].r;
+            T re, im;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const cpx<T> x = DA ? a.e[k][i] : a.e[i][k];
+                const cpx<T> y = DB ? b.e[j][k] : b.e[k][j];
+                // (x.r + i sa x.i)(y.r + i sb y.i)
+                if (k == 0 && !ACC) {
+                    re = x.r * y.r;
+                    im = (sb * x.r) * y.i;
+                } else {
+                    re = fma(x.r, y.r, k == 0 ? c.e[i][j].r : re);
+                    im = fma(sb * x.r, y.i, k == 0 ? c.e[i][j].i : im);
+                }
+                re = fma(-(sa * sb) * x.i, y.i, re);
+                im = fma(sa * x.i, y.r, im);
+            }
             c.e[i][j] = {re, im};
         }
+}
+
+// C = A B
+template <class T> __device__ __forceinline__ M3<T> mul(const M3<T> &a, const M3<T> &b)
+{
+    M3<T> c;
+    mm<false, false, false>(c, a, b);
     return c;
 }
 
@@ -82,19 +105,7 @@ template <class T> __device__ __forceinline__ M3<T> mul(const M3<T> &a, const M3
 template <class T> __device__ __forceinline__ M3<T> mul_ad(const M3<T> &a, const M3<T> &b)
 {
     M3<T> c;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            // sum_k a_ik conj(b_jk)
-            T re = a.e[i][0].r * b.e[j][0].r + a.e[i][0].i * b.e[j][0].i;
-            T im = a.e[i][0].i * b.e[j][0].r - a.e[i][0].r * b.e[j][0].i;
-            re += a.e[i][1].r * b.e[j][1].r + a.e[i][1].i * b.e[j][1].i;
-            im += a.e[i][1].i * b.e[j][1].r - a.e[i][1].r * b.e[j][1].i;
-            re += a.e[i][2].r * b.e[j][2].r + a.e[i][2].i * b.e[j][2].i;
-            im += a.e[i][2].i * b.e[j][2].r - a.e[i][2].r * b.e[j][2].i;
-            c.e[i][j] = {re, im};
-        }
+    mm<false, true, false>(c, a, b);
     return c;
 }
 
@@ -102,19 +113,7 @@ template <class T> __device__ __forceinline__ M3<T> mul_ad(const M3<T> &a, const
 template <class T> __device__ __forceinline__ M3<T> mul_da(const M3<T> &a, const M3<T> &b)
 {
     M3<T> c;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            // sum_k conj(a_ki) b_kj
-            T re = a.e[0][i].r * b.e[0][j].r + a.e[0][i].i * b.e[0][j].i;
-            T im = a.e[0][i].r * b.e[0][j].i - a.e[0][i].i * b.e[0][j].r;
-            re += a.e[1][i].r * b.e[1][j].r + a.e[1][i].i * b.e[1][j].i;
-            im += a.e[1][i].r * b.e[1][j].i - a.e[1][i].i * b.e[1][j].r;
-            re += a.e[2][i].r * b.e[2][j].r + a.e[2][i].i * b.e[2][j].i;
-            im += a.e[2][i].r * b.e[2][j].i - a.e[2][i].i * b.e[2][j].r;
-            c.e[i][j] = {re, im};
-        }
+    mm<true, false, false>(c, a, b);
     return c;
 }
 
@@ -122,19 +121,7 @@ template <class T> __device__ __forceinline__ M3<T> mul_da(const M3<T> &a, const
 template <class T> __device__ __forceinline__ M3<T> mul_dd(const M3<T> &a, const M3<T> &b)
 {
     M3<T> c;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            // sum_k conj(a_ki) conj(b_jk)
-            T re = a.e[0][i].r * b.e[j][0].r - a.e[0][i].i * b.e[j][0].i;
-            T im = -(a.e[0][i].r * b.e[j][0].i + a.e[0][i].i * b.e[j][0].r);
-            re += a.e[1][i].r * b.e[j][1].r - a.e[1][i].i * b.e[j][1].i;
-            im -= a.e[1][i].r * b.e[j][1].i + a.e[1][i].i * b.e[j][1].r;
-            re += a.e[2][i].r * b.e[j][2].r - a.e[2][i].i * b.e[j][2].i;
-            im -= a.e[2][i].r * b.e[j][2].i + a.e[2][i].i * b.e[j][2].r;
-            c.e[i][j] = {re, im};
-        }
+    mm<true, true, false>(c, a, b);
     return c;
 }
 
@@ -147,6 +134,14 @@ template <class T> __device__ __forceinline__ void add_to(M3<T> &c, const M3<T> 
             c.e[i][j].r += a.e[i][j].r;
             c.e[i][j].i += a.e[i][j].i;
         }
+}
+
+template <class T> __device__ __forceinline__ void negate(M3<T> &c)
+{
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) c.e[i][j] = {-c.e[i][j].r, -c.e[i][j].i};
 }
 
 template <class T> __device__ __forceinline__ void sub_from(M3<T> &c, const M3<T> &a)
@@ -163,12 +158,49 @@ template <class T> __device__ __forceinline__ void sub_from(M3<T> &c, const M3<T
 // Re tr(A B)
 template <class T> __device__ __forceinline__ T retrace_mul(const M3<T> &a, const M3<T> &b)
 {
-    T s = T(0);
+    T s = a.e[0][0].r * b.e[0][0].r;
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int k = 0; k < 3; k++) s += a.e[i][k].r * b.e[k][i].r - a.e[i][k].i * b.e[k][i].i;
+        for (int k = 0; k < 3; k++) {
+            if (i != 0 || k != 0) s = fma(a.e[i][k].r, b.e[k][i].r, s);
+            s = fma(-a.e[i][k].i, b.e[k][i].i, s);
+        }
     return s;
+}
+
+// Y^2 for antihermitian Y (the result is hermitian: only the upper triangle
+// is computed, the lower one is its conjugate).
+template <class T> __device__ __forceinline__ M3<T> square_antiherm(const M3<T> &y)
+{
+    M3<T> c;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        // (Y^2)_ii = -sum_k |Y_ik|^2
+        T d = -y.e[i][0].r * y.e[i][0].r;
+        d = fma(-y.e[i][0].i, y.e[i][0].i, d);
+        d = fma(-y.e[i][1].r, y.e[i][1].r, d);
+        d = fma(-y.e[i][1].i, y.e[i][1].i, d);
+        d = fma(-y.e[i][2].r, y.e[i][2].r, d);
+        d = fma(-y.e[i][2].i, y.e[i][2].i, d);
+        c.e[i][i] = {d, T(0)};
+#pragma unroll
+        for (int j = i + 1; j < 3; j++) {
+            T re = y.e[i][0].r * y.e[0][j].r, im = y.e[i][0].r * y.e[0][j].i;
+            re = fma(-y.e[i][0].i, y.e[0][j].i, re);
+            im = fma(y.e[i][0].i, y.e[0][j].r, im);
+#pragma unroll
+            for (int k = 1; k < 3; k++) {
+                re = fma(y.e[i][k].r, y.e[k][j].r, re);
+                re = fma(-y.e[i][k].i, y.e[k][j].i, re);
+                im = fma(y.e[i][k].r, y.e[k][j].i, im);
+                im = fma(y.e[i][k].i, y.e[k][j].r, im);
+            }
+            c.e[i][j] = {re, im};
+            c.e[j][i] = {re, -im};
+        }
+    }
+    return c;
 }
 
 template <class T> __device__ __forceinline__ T retrace(const M3<T> &a)
