@@ -19,17 +19,20 @@ import os
 import numpy as np
 
 __all__ = ["Lattice", "GlacxError", "library", "FP64", "FP32_LINKS", "MAIN", "FLOW",
-           "K_FLOW_STAGE", "K_MEASURE", "K_METROPOLIS", "K_PLAQUETTE", "LIB_PATH"]
+           "K_FLOW_STAGE", "K_MEASURE", "K_METROPOLIS", "K_PLAQUETTE", "K_METROPOLIS_LEARNED", "K_TRAIN",
+           "PROPOSAL_NPARAM", "LIB_PATH"]
 
 FP64, FP32_LINKS = 0, 1
 MAIN, FLOW = 0, 1
-K_FLOW_STAGE, K_MEASURE, K_METROPOLIS, K_PLAQUETTE = 0, 1, 2, 3
+K_FLOW_STAGE, K_MEASURE, K_METROPOLIS, K_PLAQUETTE, K_METROPOLIS_LEARNED, K_TRAIN = 0, 1, 2, 3, 4, 5
+PROPOSAL_NPARAM = 2468
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libglacx.so")
 _lib = None
 
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 _vp = C.c_void_p
 
 
@@ -65,6 +68,15 @@ def library():
         "glacx_plaquette": ([_vp, C.c_int, _dp], C.c_int),
         "glacx_flow": ([_vp, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp], C.c_int),
         "glacx_action_derivative": ([_vp, C.c_int, C.c_int, _dp], C.c_int),
+        "glacx_samples_reset": ([_vp, C.c_size_t], C.c_int),
+        "glacx_samples_collect": ([_vp, C.c_int, C.POINTER(C.c_size_t)], C.c_int),
+        "glacx_samples_stats": ([_vp, C.c_double, _fp, _fp], C.c_int),
+        "glacx_proposal_set": ([_vp, _fp, _fp, _fp], C.c_int),
+        "glacx_proposal_get": ([_vp, _fp, _fp, _fp], C.c_int),
+        "glacx_train": ([_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint64, _dp], C.c_int),
+        "glacx_train_grad": ([_vp, _fp, C.c_int, _fp, C.c_int, C.c_double, _dp, _fp], C.c_int),
+        "glacx_metropolis_learned": ([_vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_uint64,
+                                      C.c_uint64, _dp, _dp], C.c_int),
         "glacx_synchronize": ([_vp], C.c_int),
         "glacx_timing_enable": ([_vp, C.c_int], C.c_int),
         "glacx_timing_reset": ([_vp], C.c_int),
@@ -83,6 +95,13 @@ def _ptr(a: np.ndarray | None):
         return None
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(_dp)
+
+
+def _fptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_fp)
 
 
 class Lattice:
@@ -199,6 +218,56 @@ class Lattice:
 
     def synchronize(self):
         self._check(self._L.glacx_synchronize(self._h), "synchronize")
+
+    # ------------------------------------------------------------ learned proposal
+    def samples_reset(self, capacity):
+        self._check(self._L.glacx_samples_reset(self._h, int(capacity)), "samples_reset")
+
+    def samples_collect(self, stride=1) -> int:
+        n = C.c_size_t()
+        self._check(self._L.glacx_samples_collect(self._h, int(stride), C.byref(n)), "samples_collect")
+        return n.value
+
+    def samples_stats(self, beta):
+        mean, std = np.empty(6, np.float32), np.empty(6, np.float32)
+        self._check(self._L.glacx_samples_stats(self._h, beta, _fptr(mean), _fptr(std)), "samples_stats")
+        return mean, std
+
+    def proposal_set(self, params, mean, std):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == PROPOSAL_NPARAM
+        self._check(self._L.glacx_proposal_set(self._h, _fptr(p), _fptr(np.ascontiguousarray(mean, np.float32)),
+                                               _fptr(np.ascontiguousarray(std, np.float32))), "proposal_set")
+
+    def proposal_get(self):
+        p = np.empty(PROPOSAL_NPARAM, np.float32)
+        mean, std = np.empty(6, np.float32), np.empty(6, np.float32)
+        self._check(self._L.glacx_proposal_get(self._h, _fptr(p), _fptr(mean), _fptr(std)), "proposal_get")
+        return p, mean, std
+
+    def train(self, steps, batch=4096, draws=4, lr=1e-3, beta=6.0, seed=1):
+        loss = np.empty(steps)
+        self._check(self._L.glacx_train(self._h, steps, batch, draws, lr, beta, seed, _ptr(loss)), "train")
+        return loss
+
+    def train_grad(self, samples, z, beta):
+        smp = np.ascontiguousarray(samples, dtype=np.float32)
+        zz = np.ascontiguousarray(z, dtype=np.float32)
+        n, draws = zz.shape[0], zz.shape[1]
+        assert smp.shape == (n, 36)
+        loss = C.c_double()
+        grad = np.empty(PROPOSAL_NPARAM, np.float32)
+        self._check(self._L.glacx_train_grad(self._h, _fptr(smp), n, _fptr(zz), draws, beta, C.byref(loss),
+                                             _fptr(grad)), "train_grad")
+        return loss.value, grad
+
+    def metropolis_learned(self, n_sweeps, beta, eps=0.24, n_hit=10, reunit_every=1, seed=1, sweep_offset=0,
+                           plaquette=False):
+        acc = np.empty(n_sweeps)
+        plaq = np.empty(n_sweeps) if plaquette else None
+        self._check(self._L.glacx_metropolis_learned(self._h, n_sweeps, beta, eps, n_hit, reunit_every, seed,
+                                                     sweep_offset, _ptr(acc), _ptr(plaq)), "metropolis_learned")
+        return (acc, plaq) if plaquette else acc
 
     # ------------------------------------------------------------ timing
     def timing(self, on=True):

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -37,8 +38,19 @@ struct glacx_ctx {
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> timed;
-    long long t_launches[GLACX_K_COUNT] = {0, 0, 0, 0};
-    double t_ms[GLACX_K_COUNT] = {0, 0, 0, 0};
+    long long t_launches[GLACX_K_COUNT] = {};
+    double t_ms[GLACX_K_COUNT] = {};
+    // learned proposal
+    float *samples = nullptr;        // column-major (A, U) pairs
+    size_t samples_cap = 0, samples_n = 0;
+    float *params = nullptr;         // FP32 master parameters, then Adam m, v, gradient
+    float *adam_m = nullptr, *adam_v = nullptr, *grad = nullptr;
+    unsigned short *params_bf16 = nullptr;  // inference copy
+    float *mean_std = nullptr;       // 12 floats
+    bool proposal_ready = false;
+    long long adam_t = 0;
+    float *inter = nullptr;          // per-sample training columns
+    size_t inter_len = 0;
 };
 
 namespace {
@@ -194,6 +206,53 @@ template <class T> int run_metropolis_sweep(glacx_ctx *c, const MetroParams &p, 
     return GLACX_OK;
 }
 
+int ensure_proposal_buffers(glacx_ctx *c)
+{
+    if (c->params) return GLACX_OK;
+    void *p = nullptr;
+    int rc = dev_alloc(c, &p, sizeof(float) * 4 * kProposalParams);  // params, m, v, grad
+    if (rc) return rc;
+    c->params = (float *)p;
+    c->adam_m = c->params + kProposalParams;
+    c->adam_v = c->adam_m + kProposalParams;
+    c->grad = c->adam_v + kProposalParams;
+    HIP_TRY(c, hipMemsetAsync(c->params, 0, sizeof(float) * 4 * kProposalParams, c->stream));
+    if ((rc = dev_alloc(c, &p, sizeof(unsigned short) * kProposalParams))) return rc;
+    c->params_bf16 = (unsigned short *)p;
+    if ((rc = dev_alloc(c, &p, sizeof(float) * 12))) return rc;
+    c->mean_std = (float *)p;
+    return GLACX_OK;
+}
+
+template <class T> int run_metropolis_learned_sweep(glacx_ctx *c, const MetroParams &p, unsigned long long *count)
+{
+    int rc;
+    for (int mu = 0; mu < 4; mu++)
+        for (int parity = 0; parity < 2; parity++) {
+            hipEvent_t e0 = nullptr;
+            if ((rc = timed_begin(c, &e0))) return rc;
+            launch_metropolis_learned<T>(c->field[GLACX_FIELD_MAIN], c->geom, mu, parity, p, c->params_bf16,
+                                         c->mean_std, count, c->stream);
+            CHECK_LAUNCH(c);
+            if ((rc = timed_end(c, GLACX_K_METROPOLIS_LEARNED, e0))) return rc;
+        }
+    return GLACX_OK;
+}
+
+template <class T> int run_collect(glacx_ctx *c, int stride)
+{
+    const size_t per = (size_t)((c->geom.vh + stride - 1) / stride);
+    for (int mu = 0; mu < 4; mu++)
+        for (int parity = 0; parity < 2; parity++) {
+            if (c->samples_n >= c->samples_cap) return GLACX_OK;
+            launch_collect<T>(c->field[GLACX_FIELD_MAIN], c->geom, mu, parity, stride, c->samples, c->samples_cap,
+                              c->samples_n, c->stream);
+            CHECK_LAUNCH(c);
+            c->samples_n = std::min(c->samples_cap, c->samples_n + per);
+        }
+    return GLACX_OK;
+}
+
 }  // namespace
 
 #define CTX_CHECK(c)                                                                                    \
@@ -248,7 +307,8 @@ int glacx_destroy(glacx_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (void *p : {c->field[0], c->field[1], c->work, c->acc, (void *)c->partials, (void *)c->results,
-                    (void *)c->counts, (void *)c->stage})
+                    (void *)c->counts, (void *)c->stage, (void *)c->samples, (void *)c->params,
+                    (void *)c->params_bf16, (void *)c->mean_std, (void *)c->inter})
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -480,6 +540,209 @@ int glacx_action_derivative(glacx_ctx *c, int field, int mu, double *z_out)
     HIP_TRY(c, hipMemcpyAsync(z_out, c->stage, (size_t)c->geom.v * 18 * sizeof(double), hipMemcpyDeviceToHost,
                               c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GLACX_OK;
+}
+
+int glacx_samples_reset(glacx_ctx *c, size_t capacity)
+{
+    CTX_CHECK(c);
+    if (capacity == 0) return fail(c, GLACX_E_ARG, "samples_reset: zero capacity");
+    if (c->samples) {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        HIP_TRY(c, hipFree(c->samples));
+        c->device_bytes -= sizeof(float) * 36 * c->samples_cap;
+        c->samples = nullptr;
+    }
+    void *p = nullptr;
+    int rc = dev_alloc(c, &p, sizeof(float) * 36 * capacity);
+    if (rc) return rc;
+    c->samples = (float *)p;
+    c->samples_cap = capacity;
+    c->samples_n = 0;
+    return GLACX_OK;
+}
+
+int glacx_samples_collect(glacx_ctx *c, int stride, size_t *count)
+{
+    CTX_CHECK(c);
+    if (!c->samples || stride < 1) return fail(c, GLACX_E_STATE, "samples_collect: no sample buffer or bad stride");
+    int rc = DISPATCH(c, run_collect, c, stride);
+    if (rc) return rc;
+    if (count) *count = c->samples_n;
+    return GLACX_OK;
+}
+
+int glacx_samples_stats(glacx_ctx *c, double beta, float mean[6], float stdv[6])
+{
+    CTX_CHECK(c);
+    if (!c->samples || c->samples_n == 0) return fail(c, GLACX_E_STATE, "samples_stats: no samples");
+    int rc = ensure_proposal_buffers(c);
+    if (rc) return rc;
+    if ((rc = ensure(c, &c->partials, &c->partials_len, (size_t)12 * feature_moments_blocks(c->samples_n)))) return rc;
+    launch_feature_moments(c->samples, c->samples_cap, c->samples_n, beta, c->partials, c->mean_std, c->stream);
+    CHECK_LAUNCH(c);
+    float h[12];
+    HIP_TRY(c, hipMemcpyAsync(h, c->mean_std, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (mean) std::memcpy(mean, h, 6 * sizeof(float));
+    if (stdv) std::memcpy(stdv, h + 6, 6 * sizeof(float));
+    return GLACX_OK;
+}
+
+int glacx_proposal_set(glacx_ctx *c, const float *params, const float mean[6], const float stdv[6])
+{
+    CTX_CHECK(c);
+    if (!params || !mean || !stdv) return fail(c, GLACX_E_ARG, "proposal_set: null pointer");
+    for (int k = 0; k < 6; k++)
+        if (!(stdv[k] > 0.f)) return fail(c, GLACX_E_ARG, "proposal_set: std must be positive");
+    int rc = ensure_proposal_buffers(c);
+    if (rc) return rc;
+    float ms[12];
+    std::memcpy(ms, mean, 6 * sizeof(float));
+    std::memcpy(ms + 6, stdv, 6 * sizeof(float));
+    HIP_TRY(c, hipMemcpyAsync(c->params, params, sizeof(float) * kProposalParams, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->mean_std, ms, sizeof ms, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->adam_m, 0, sizeof(float) * 2 * kProposalParams, c->stream));
+    launch_to_bf16(c->params, c->params_bf16, c->stream);
+    CHECK_LAUNCH(c);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->adam_t = 0;
+    c->proposal_ready = true;
+    return GLACX_OK;
+}
+
+int glacx_proposal_get(glacx_ctx *c, float *params, float mean[6], float stdv[6])
+{
+    CTX_CHECK(c);
+    if (!c->proposal_ready) return fail(c, GLACX_E_STATE, "proposal_get: no proposal set");
+    float ms[12];
+    if (params)
+        HIP_TRY(c, hipMemcpyAsync(params, c->params, sizeof(float) * kProposalParams, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(ms, c->mean_std, sizeof ms, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (mean) std::memcpy(mean, ms, 6 * sizeof(float));
+    if (stdv) std::memcpy(stdv, ms + 6, 6 * sizeof(float));
+    return GLACX_OK;
+}
+
+int glacx_train(glacx_ctx *c, int steps, int batch, int draws, double lr, double beta, uint64_t seed, double *loss)
+{
+    CTX_CHECK(c);
+    if (!c->proposal_ready) return fail(c, GLACX_E_STATE, "train: set initial parameters first");
+    if (!c->samples || c->samples_n == 0) return fail(c, GLACX_E_STATE, "train: no samples");
+    if (steps < 0 || batch < 1 || draws < 1 || !(lr > 0.0)) return fail(c, GLACX_E_ARG, "train: bad arguments");
+    int rc;
+    const int nblk = (batch + kBlock - 1) / kBlock;
+    if ((rc = ensure(c, &c->inter, &c->inter_len, (size_t)kTrainInterCols * batch))) return rc;
+    if ((rc = ensure(c, &c->partials, &c->partials_len, (size_t)nblk * steps))) return rc;
+    for (int i = 0; i < steps; i++) {
+        hipEvent_t e0 = nullptr;
+        if ((rc = timed_begin(c, &e0))) return rc;
+        c->adam_t++;
+        launch_train_step(c->samples, c->samples_cap, c->samples_n, batch, draws, seed, (uint64_t)c->adam_t, beta,
+                          c->params, c->mean_std, nullptr, c->inter, c->partials + (size_t)i * nblk, c->grad,
+                          c->adam_m, c->adam_v, (float)lr, (int)c->adam_t, c->params_bf16, c->stream);
+        CHECK_LAUNCH(c);
+        if ((rc = timed_end(c, GLACX_K_TRAIN, e0))) return rc;
+    }
+    if (loss && steps > 0) {
+        std::vector<double> h((size_t)nblk * steps);
+        HIP_TRY(c, hipMemcpyAsync(h.data(), c->partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        for (int i = 0; i < steps; i++) {
+            double s = 0.0;
+            for (int b = 0; b < nblk; b++) s += h[(size_t)i * nblk + b];
+            loss[i] = s;
+        }
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GLACX_OK;
+}
+
+int glacx_train_grad(glacx_ctx *c, const float *samples, int n, const float *z, int draws, double beta, double *loss,
+                     float *grad)
+{
+    CTX_CHECK(c);
+    if (!c->proposal_ready) return fail(c, GLACX_E_STATE, "train_grad: set parameters first");
+    if (!samples || !z || n < 1 || draws < 1) return fail(c, GLACX_E_ARG, "train_grad: bad arguments");
+    // column-major copy of the host samples, then the draws
+    std::vector<float> cols((size_t)36 * n);
+    for (int s = 0; s < n; s++)
+        for (int k = 0; k < 36; k++) cols[(size_t)k * n + s] = samples[(size_t)s * 36 + k];
+    float *dsmp = nullptr, *dz = nullptr;
+    HIP_TRY(c, hipMalloc((void **)&dsmp, cols.size() * sizeof(float)));
+    const size_t nz = (size_t)n * draws * 8;
+    hipError_t e = hipMalloc((void **)&dz, nz * sizeof(float));
+    if (e != hipSuccess) {
+        (void)hipFree(dsmp);
+        return fail(c, GLACX_E_HIP, "train_grad: hipMalloc failed");
+    }
+    int rc = GLACX_OK;
+    const int nblk = (n + kBlock - 1) / kBlock;
+    do {
+        if ((rc = ensure(c, &c->inter, &c->inter_len, (size_t)kTrainInterCols * n))) break;
+        if ((rc = ensure(c, &c->partials, &c->partials_len, (size_t)nblk))) break;
+        if (hipMemcpyAsync(dsmp, cols.data(), cols.size() * sizeof(float), hipMemcpyHostToDevice, c->stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(dz, z, nz * sizeof(float), hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+            rc = fail(c, GLACX_E_HIP, "train_grad: copy failed");
+            break;
+        }
+        launch_train_step(dsmp, (size_t)n, (size_t)n, n, draws, 0, 0, beta, c->params, c->mean_std, dz, c->inter,
+                          c->partials, c->grad, c->adam_m, c->adam_v, 0.f, 0, c->params_bf16, c->stream);
+        if (hipGetLastError() != hipSuccess) {
+            rc = fail(c, GLACX_E_HIP, "train_grad: launch failed");
+            break;
+        }
+        std::vector<double> h(nblk);
+        if (hipMemcpyAsync(h.data(), c->partials, nblk * sizeof(double), hipMemcpyDeviceToHost, c->stream) !=
+                hipSuccess ||
+            (grad && hipMemcpyAsync(grad, c->grad, sizeof(float) * kProposalParams, hipMemcpyDeviceToHost,
+                                    c->stream) != hipSuccess) ||
+            hipStreamSynchronize(c->stream) != hipSuccess) {
+            rc = fail(c, GLACX_E_HIP, "train_grad: readback failed");
+            break;
+        }
+        double s = 0.0;
+        for (double v : h) s += v;
+        if (loss) *loss = s;
+    } while (0);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(dsmp);
+    (void)hipFree(dz);
+    return rc;
+}
+
+int glacx_metropolis_learned(glacx_ctx *c, int n_sweeps, double beta, double eps, int n_hit, int reunit_every,
+                             uint64_t seed, uint64_t sweep_offset, double *accept, double *plaq)
+{
+    CTX_CHECK(c);
+    if (!c->proposal_ready) return fail(c, GLACX_E_STATE, "metropolis_learned: no proposal set");
+    if (n_sweeps < 0 || n_hit < 1 || !(eps > 0.0 && eps < 1.0))
+        return fail(c, GLACX_E_ARG, "metropolis_learned: need n_sweeps >= 0, n_hit >= 1, 0 < eps < 1");
+    if (n_sweeps == 0) return GLACX_OK;
+    int rc = ensure(c, &c->counts, &c->counts_len, (size_t)n_sweeps);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemsetAsync(c->counts, 0, sizeof(unsigned long long) * n_sweeps, c->stream));
+    if (plaq && (rc = ensure(c, &c->results, &c->results_len, (size_t)n_sweeps))) return rc;
+    for (int i = 0; i < n_sweeps; i++) {
+        MetroParams p;
+        p.beta = beta;
+        p.eps = eps;
+        p.n_hit = n_hit;
+        p.sweep = sweep_offset + (uint64_t)i;
+        p.reunit = reunit_every > 0 && (p.sweep % (uint64_t)reunit_every) == 0;
+        p.seed = seed;
+        if ((rc = DISPATCH(c, run_metropolis_learned_sweep, c, p, c->counts + i))) return rc;
+        if (plaq && (rc = DISPATCH(c, run_plaquette, c, c->field[GLACX_FIELD_MAIN], c->results + i))) return rc;
+    }
+    std::vector<unsigned long long> counts(n_sweeps);
+    HIP_TRY(c, hipMemcpyAsync(counts.data(), c->counts, sizeof(unsigned long long) * n_sweeps, hipMemcpyDeviceToHost,
+                              c->stream));
+    if (plaq) HIP_TRY(c, hipMemcpyAsync(plaq, c->results, sizeof(double) * n_sweeps, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (accept)
+        for (int i = 0; i < n_sweeps; i++) accept[i] = (double)counts[i] / (4.0 * n_hit * (double)c->geom.v);
     return GLACX_OK;
 }
 

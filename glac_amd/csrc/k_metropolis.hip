@@ -19,77 +19,9 @@
 // counter-based stream (rng.hpp), 13 centred ones for the proposal and one
 // open one for the test per hit, in that order.
 #include "kernels.hpp"
-#include "rng.hpp"
+#include "proposal.hpp"
 
 namespace glacx {
-
-// SU(2) near unity from 4 centred uniforms; returns (a, b) with
-// matrix [[a, b], [-conj(b), conj(a)]], a = x0 + i x3, b = x2 + i x1.
-template <class T>
-__device__ __forceinline__ void su2_near_unity(const T r0, const T r1, const T r2, const T r3, T x0abs, T eps,
-                                               cpx<T> &a, cpx<T> &b)
-{
-    T x0 = r0 < T(0) ? -x0abs : x0abs;
-    const T sc = eps * rsqrt(fma(r1, r1, fma(r2, r2, r3 * r3)));  // eps / |r|
-    T x1 = r1 * sc, x2 = r2 * sc, x3 = r3 * sc;
-    // |x| = 1 up to rounding (x0^2 = 1 - eps^2); renormalise as GLAC does
-    const T inv = rsqrt(fma(x0, x0, fma(x1, x1, fma(x2, x2, x3 * x3))));
-    x0 *= inv;
-    x1 *= inv;
-    x2 *= inv;
-    x3 *= inv;
-    a = {x0, x3};
-    b = {x2, x1};
-}
-
-// X = R S T with r in block (0,1), s in block (0,2), t in block (1,2).
-template <class T>
-__device__ __forceinline__ M3<T> rst_product(cpx<T> ra, cpx<T> rb, cpx<T> sa, cpx<T> sb, cpx<T> ta, cpx<T> tb)
-{
-    // 2x2 entries: m00 = a, m01 = b, m10 = -conj(b), m11 = conj(a)
-    const cpx<T> r00 = ra, r01 = rb, r10 = {-rb.r, rb.i}, r11 = {ra.r, -ra.i};
-    const cpx<T> s00 = sa, s01 = sb, s10 = {-sb.r, sb.i}, s11 = {sa.r, -sa.i};
-    const cpx<T> t00 = ta, t01 = tb, t10 = {-tb.r, tb.i}, t11 = {ta.r, -ta.i};
-    const cpx<T> r00s01 = cmul(r00, s01), r10s01 = cmul(r10, s01);
-    M3<T> x;
-    x.e[0][0] = cmul(r00, s00);
-    {
-        const cpx<T> u = cmul(r01, t00), v = cmul(r00s01, t10);
-        x.e[0][1] = {u.r + v.r, u.i + v.i};
-    }
-    {
-        const cpx<T> u = cmul(r01, t01), v = cmul(r00s01, t11);
-        x.e[0][2] = {u.r + v.r, u.i + v.i};
-    }
-    x.e[1][0] = cmul(r10, s00);
-    {
-        const cpx<T> u = cmul(r11, t00), v = cmul(r10s01, t10);
-        x.e[1][1] = {u.r + v.r, u.i + v.i};
-    }
-    {
-        const cpx<T> u = cmul(r11, t01), v = cmul(r10s01, t11);
-        x.e[1][2] = {u.r + v.r, u.i + v.i};
-    }
-    x.e[2][0] = s10;
-    x.e[2][1] = cmul(s11, t10);
-    x.e[2][2] = cmul(s11, t11);
-    return x;
-}
-
-// exp(-dS): FP64 libm exp for FP64 links; the hardware base-2 exponential for
-// FP32 links, whose dS already carries FP32 rounding.
-__device__ __forceinline__ double accept_prob(double ds) { return exp(-ds); }
-__device__ __forceinline__ double accept_prob(float ds) { return (double)__expf(-ds); }
-
-template <class T> __device__ __forceinline__ M3<T> dagger(const M3<T> &m)
-{
-    M3<T> d;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) d.e[i][j] = {m.e[j][i].r, -m.e[j][i].i};
-    return d;
-}
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void metropolis_kernel(typename vec2<T>::type *__restrict__ f, Geom g, int mu,
@@ -112,22 +44,7 @@ __global__ __launch_bounds__(kBlock) void metropolis_kernel(typename vec2<T>::ty
         int accepted = 0;
 #pragma unroll 1
         for (int hit = 0; hit < prm.n_hit; hit++) {
-            cpx<T> ra, rb, sa, sb, ta, tb;
-            {
-                const T q0 = rs.centered<T>(), q1 = rs.centered<T>(), q2 = rs.centered<T>(), q3 = rs.centered<T>();
-                su2_near_unity(q0, q1, q2, q3, x0abs, eps, ra, rb);
-            }
-            {
-                const T q0 = rs.centered<T>(), q1 = rs.centered<T>(), q2 = rs.centered<T>(), q3 = rs.centered<T>();
-                su2_near_unity(q0, q1, q2, q3, x0abs, eps, sa, sb);
-            }
-            {
-                const T q0 = rs.centered<T>(), q1 = rs.centered<T>(), q2 = rs.centered<T>(), q3 = rs.centered<T>();
-                su2_near_unity(q0, q1, q2, q3, x0abs, eps, ta, tb);
-            }
-            const T coin = rs.centered<T>();
-            M3<T> xm = rst_product(ra, rb, sa, sb, ta, tb);
-            if (coin < T(0)) xm = dagger(xm);
+            const M3<T> xm = rst_proposal<T>(rs, x0abs, eps);
             const T ds = T(-bf) * (retrace_mul(xm, w) - trw);
             if (accept_prob(ds) > rs.open()) {
                 u = mul(xm, u);

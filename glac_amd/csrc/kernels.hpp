@@ -81,4 +81,28 @@ template <class T>
 void launch_metropolis(void *f, const Geom &g, int mu, int parity, const MetroParams &p,
                        unsigned long long *count, hipStream_t s);
 
+// ---- learned proposal (k_learned.hip) ----------------------------------------
+// Samples are stored column-major: column c (0..35: A then U, 18 reals each)
+// of sample s at out[c * cap + s].
+template <class T>
+void launch_collect(const void *f, const Geom &g, int mu, int parity, int stride, float *out, size_t cap, size_t base,
+                    hipStream_t s);
+int feature_moments_blocks(size_t n);
+void launch_feature_moments(const float *smp, size_t cap, size_t n, double beta, double *partials, float *mean_std,
+                            hipStream_t s);
+// One training step over `batch` samples (drawn from the first n_samples, or
+// the first `batch` in order with draws read from zin when zin != nullptr);
+// adam_t > 0 applies an Adam update (bias corrections for step adam_t).
+void launch_train_step(const float *smp, size_t cap, size_t n_samples, int batch, int draws, uint64_t seed,
+                       uint64_t step, double beta, float *params, const float *mean_std, const float *zin,
+                       float *inter, double *loss_partials, float *grad, float *m, float *v, float lr, int adam_t,
+                       unsigned short *bf16, hipStream_t s);
+void launch_to_bf16(const float *params, unsigned short *bf16, hipStream_t s);
+template <class T>
+void launch_metropolis_learned(void *f, const Geom &g, int mu, int parity, const MetroParams &p,
+                               const unsigned short *wbf16, const float *mean_std, unsigned long long *count,
+                               hipStream_t s);
+constexpr int kTrainInterCols = 170;  // per-sample columns kept by the training step
+constexpr int kProposalParams = 2468;
+
 } // namespace glacx

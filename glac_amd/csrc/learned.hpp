@@ -1,0 +1,153 @@
+// learned.hpp -- the staple-conditioned learned Metropolis proposal.
+//
+// No GLAC counterpart: it replaces the fixed-eps RST proposal
+// (GLAC/math/matrices/su3matrixgenerator.h:170-321) when a trained network
+// is loaded.  For a link with staple A:
+//   features f(A) = (tr AA^dag, tr (AA^dag)^2, |det A|^2, Re det A, Im det A, beta),
+//       all invariant under A -> g A h^dagger, standardised with (mean, std);
+//   MLP 6 -> 32 -> 32 -> 36 (leaky ReLU, slope 0.01), weights in BF16,
+//       FP32 accumulation;
+//   output o -> lower-triangular L: L_aa = exp(o_a), L_ab = o[8 + a(a-1)/2 + b];
+//   per hit w = L z with z symmetric (z_a = (c1 + c2) sqrt 6, c uniform on the
+//       2^24 midpoints of [-1/2, 1/2)), X = exp(i sum_a w_a lambda_a / 2),
+//       U' = X U, accepted with GLAC's exp(-dS) > u.
+// L depends on A only and z -> -z is a symmetry of the draw, so the
+// proposal density of X equals that of X^dagger: exact accept/reject keeps
+// detailed balance with respect to exp(-S_W).  If a standardised feature
+// leaves [-kOutOfRange, kOutOfRange] (a staple unlike any seen in training)
+// the link falls back to the fixed-eps RST proposal; that choice also
+// depends on A only.
+#pragma once
+#include "expsu3.hpp"
+#include "lattice.hpp"
+
+namespace glacx {
+
+constexpr int kMlpIn = 6, kMlpH1 = 32, kMlpH2 = 32, kMlpOut = 36;
+constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kMlpH1 * kMlpIn, kOffW2 = kOffB1 + kMlpH1,
+              kOffB2 = kOffW2 + kMlpH2 * kMlpH1, kOffW3 = kOffB2 + kMlpH2, kOffB3 = kOffW3 + kMlpOut * kMlpH2,
+              kNParam = kOffB3 + kMlpOut;
+static_assert(kNParam == 2468, "parameter packing");
+constexpr float kOutOfRange = 8.0f;
+
+__device__ __forceinline__ float lrelu(float v) { return v > 0.f ? v : 0.01f * v; }
+
+// features of the staple (FP64 accumulation of the invariants)
+template <class T> __device__ __forceinline__ void staple_features(const M3<T> &a, double beta, float f[6])
+{
+    const M3<double> A = convert<double>(a);
+    const M3<double> B = mul_ad(A, A);  // A A^dagger, hermitian
+    double trb = B.e[0][0].r + B.e[1][1].r + B.e[2][2].r, trb2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) trb2 += B.e[i][j].r * B.e[i][j].r + B.e[i][j].i * B.e[i][j].i;
+    // det A by cofactors of row 0
+    double dr = 0.0, di = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+        const cpx<double> m1 = cmul(A.e[1][j1], A.e[2][j2]), m2 = cmul(A.e[1][j2], A.e[2][j1]);
+        const cpx<double> mnr = {m1.r - m2.r, m1.i - m2.i};
+        const cpx<double> t = cmul(A.e[0][j], mnr);
+        dr += t.r;
+        di += t.i;
+    }
+    f[0] = (float)trb;
+    f[1] = (float)trb2;
+    f[2] = (float)(dr * dr + di * di);
+    f[3] = (float)dr;
+    f[4] = (float)di;
+    f[5] = (float)beta;
+}
+
+// Network forward pass with parameters P (FP32 values, e.g. in LDS).
+// Returns false when a standardised feature is out of range.
+__device__ __forceinline__ bool mlp_forward(const float *P, const float *mean, const float *stdv, const float f[6],
+                                            float o[36])
+{
+    float x[kMlpIn];
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < kMlpIn; k++) {
+        x[k] = (f[k] - mean[k]) / stdv[k];
+        ok = ok && fabsf(x[k]) <= kOutOfRange;
+    }
+    float h1[kMlpH1];
+#pragma unroll
+    for (int i = 0; i < kMlpH1; i++) {
+        float s = P[kOffB1 + i];
+#pragma unroll
+        for (int k = 0; k < kMlpIn; k++) s = fmaf(P[kOffW1 + i * kMlpIn + k], x[k], s);
+        h1[i] = lrelu(s);
+    }
+    float h2[kMlpH2];
+#pragma unroll
+    for (int i = 0; i < kMlpH2; i++) {
+        float s = P[kOffB2 + i];
+#pragma unroll
+        for (int k = 0; k < kMlpH1; k++) s = fmaf(P[kOffW2 + i * kMlpH1 + k], h1[k], s);
+        h2[i] = lrelu(s);
+    }
+#pragma unroll
+    for (int i = 0; i < kMlpOut; i++) {
+        float s = P[kOffB3 + i];
+#pragma unroll
+        for (int k = 0; k < kMlpH2; k++) s = fmaf(P[kOffW3 + i * kMlpH2 + k], h2[k], s);
+        o[i] = s;
+    }
+    return ok;
+}
+
+// packed lower triangle, row a holds L[a][0..a] at a(a+1)/2
+__device__ __forceinline__ void cholesky_from_output(const float o[36], float L[36])
+{
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+#pragma unroll
+        for (int b = 0; b < a; b++) L[a * (a + 1) / 2 + b] = o[8 + a * (a - 1) / 2 + b];
+        L[a * (a + 1) / 2 + a] = __expf(o[a]);
+    }
+}
+
+// Y = i sum_a w_a lambda_a / 2 in compact form
+template <class T> __device__ __forceinline__ su3alg<T> algebra_from_omega(const float w[8])
+{
+    const float r3 = 0.57735026918962576f;
+    su3alg<T> y;
+    y.a0 = T(0.5f * fmaf(r3, w[7], w[2]));
+    y.a1 = T(0.5f * fmaf(r3, w[7], -w[2]));
+    y.p[0] = T(0.5f * w[1]);
+    y.p[1] = T(0.5f * w[0]);
+    y.p[2] = T(0.5f * w[4]);
+    y.p[3] = T(0.5f * w[3]);
+    y.p[4] = T(0.5f * w[6]);
+    y.p[5] = T(0.5f * w[5]);
+    return y;
+}
+
+// symmetric unit-variance draw from a stream (c uniform on the 2^24 midpoints of [-1/2, 1/2))
+template <class S> __device__ __forceinline__ float sym_draw(S &rs)
+{
+    const float c1 = (float)(rs.next() >> 8) * 0x1p-24f - (0.5f - 0x1p-25f);
+    const float c2 = (float)(rs.next() >> 8) * 0x1p-24f - (0.5f - 0x1p-25f);
+    return (c1 + c2) * 2.4494897427831781f;
+}
+
+// Re tr(i lambda_a / 2 N) for a = 0..7
+template <class T> __device__ __forceinline__ void project_algebra(const M3<T> &n, float g[8])
+{
+    const float r3 = 0.57735026918962576f;
+    g[0] = -0.5f * float(n.e[0][1].i + n.e[1][0].i);
+    g[1] = 0.5f * float(n.e[1][0].r - n.e[0][1].r);
+    g[2] = -0.5f * float(n.e[0][0].i - n.e[1][1].i);
+    g[3] = -0.5f * float(n.e[0][2].i + n.e[2][0].i);
+    g[4] = 0.5f * float(n.e[2][0].r - n.e[0][2].r);
+    g[5] = -0.5f * float(n.e[1][2].i + n.e[2][1].i);
+    g[6] = 0.5f * float(n.e[2][1].r - n.e[1][2].r);
+    g[7] = -0.5f * r3 * float(n.e[0][0].i + n.e[1][1].i - 2.f * n.e[2][2].i);
+}
+
+__device__ __forceinline__ float bf16_to_float(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+
+} // namespace glacx

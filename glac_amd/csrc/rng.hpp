@@ -14,7 +14,7 @@
 
 namespace glacx {
 
-enum StreamTag : uint32_t { STREAM_METROPOLIS = 1, STREAM_HOT_START = 2, STREAM_LEARNED = 3 };
+enum StreamTag : uint32_t { STREAM_METROPOLIS = 1, STREAM_HOT_START = 2, STREAM_LEARNED = 3, STREAM_TRAIN = 4 };
 
 __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
 {

@@ -1,0 +1,326 @@
+// glac_gpu.hpp -- GLAC's Action / Flow / Correlator / System interfaces for the
+// hot path, implemented over the C ABI of libglacx.so (include/glacx.h).
+//
+// Header-only templates, so they instantiate against GLAC's own types: a
+// lattice type L is anything with GLAC's public Lattice<SU3> shape
+// (GLAC/math/lattice.h:23-29): `std::vector<SU3> m_sites`,
+// `std::vector<unsigned int> m_dim`, `unsigned long m_latticeSize`, where
+// SU3 is 18 doubles (`double mat[18]`, GLAC/math/matrices/su3.h:22).  A
+// gauge field is GLAC's `L lattice[4]` (GLAC/system.cpp:158-161).
+//
+// Residency.  GLAC calls Flow::flowField(lattice) and then
+// Correlator::calculate(lattice) every flow step (GLAC/system.cpp:494-508).
+// The Device keeps each bound host field resident on the GPU and tracks which
+// copy is newer, so such a loop moves no lattice data between host and device:
+//   - the first use of a host field uploads it (or after markHostModified);
+//   - device-side updates (flow, Metropolis) mark the device copy newer;
+//   - syncHost() downloads a device-newer field before host code reads it
+//     (e.g. before GLAC's IO::FieldIO::writeFieldToFile).
+//
+// Errors follow GLAC's convention of a fatal, printed exit
+// (Parallel::Communicator::MPIExit, GLAC/parallelization/communicator.cpp:
+// 526-533), except that the exit status is non-zero: GLAC exits with 0.
+#pragma once
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "../../include/glacx.h"
+
+namespace glac_gpu {
+
+[[noreturn]] inline void fatal(const std::string &msg)
+{
+    std::fprintf(stderr, "glac_gpu: %s\n", msg.c_str());
+    std::exit(EXIT_FAILURE);
+}
+
+// ---------------------------------------------------------------- device ---
+class Device {
+public:
+    Device(const std::vector<unsigned int> &dims, int precision = GLACX_FP64, int device = 0)
+    {
+        if (dims.size() != 4) fatal("lattice needs 4 extents");
+        for (int k = 0; k < 4; k++) m_dims[k] = (int)dims[k];
+        m_volume = (unsigned long)m_dims[0] * m_dims[1] * m_dims[2] * m_dims[3];
+        const int rc = glacx_create(m_dims, precision, device, &m_ctx);
+        if (rc != GLACX_OK)
+            fatal(std::string("glacx_create failed: ") + (m_ctx ? glacx_last_error(m_ctx) : "bad lattice extents"));
+    }
+    ~Device() { glacx_destroy(m_ctx); }
+    Device(const Device &) = delete;
+    Device &operator=(const Device &) = delete;
+
+    glacx_ctx *ctx() { return m_ctx; }
+    const int *dims() const { return m_dims; }
+    unsigned long volume() const { return m_volume; }
+
+    void check(int rc, const char *what)
+    {
+        if (rc != GLACX_OK) fatal(std::string(what) + ": " + glacx_last_error(m_ctx));
+    }
+
+    // Device field holding `lattice` (uploading it if needed).
+    template <class L> int acquire(L *lattice)
+    {
+        Binding &b = bind(lattice);
+        if (b.host_newer) {
+            upload(lattice, b.field);
+            b.host_newer = false;
+            b.device_newer = false;
+        }
+        return b.field;
+    }
+    // The device copy of `lattice` was just modified on the GPU.
+    template <class L> void markDeviceModified(L *lattice) { bind(lattice).device_newer = true; }
+    // Host code modified `lattice`: the next device use uploads it.
+    template <class L> void markHostModified(L *lattice)
+    {
+        Binding &b = bind(lattice);
+        b.host_newer = true;
+        b.device_newer = false;
+    }
+    // Bring the host copy up to date.
+    template <class L> void syncHost(L *lattice)
+    {
+        Binding &b = bind(lattice);
+        if (b.device_newer) {
+            double *links[4];
+            for (int mu = 0; mu < 4; mu++) links[mu] = sites(lattice[mu]);
+            check(glacx_download_field(m_ctx, b.field, links), "download");
+            b.device_newer = false;
+        }
+    }
+
+private:
+    struct Binding {
+        const void *host = nullptr;
+        int field = 0;
+        bool host_newer = true;
+        bool device_newer = false;
+    };
+
+    template <class Lmu> double *sites(Lmu &l)
+    {
+        if (l.m_latticeSize != m_volume || l.m_sites.size() != m_volume) fatal("lattice size does not match the device");
+        static_assert(sizeof(l.m_sites[0]) == 18 * sizeof(double), "SU3 must be 18 doubles");
+        return reinterpret_cast<double *>(&l.m_sites[0]);
+    }
+
+    template <class L> Binding &bind(L *lattice)
+    {
+        for (Binding &b : m_bind)
+            if (b.host == lattice) return b;
+        if (m_bind.size() == 2) {
+            // evict the older binding, writing it back first
+            Binding &old = m_bind.front();
+            if (old.device_newer) fatal("two device fields are in use and the evicted one is newer on the device; "
+                                        "call syncHost() first");
+            m_bind.erase(m_bind.begin());
+        }
+        Binding b;
+        b.host = lattice;
+        b.field = m_bind.empty() ? GLACX_FIELD_MAIN : (m_bind[0].field == GLACX_FIELD_MAIN ? GLACX_FIELD_FLOW
+                                                                                             : GLACX_FIELD_MAIN);
+        m_bind.push_back(b);
+        return m_bind.back();
+    }
+
+    template <class L> void upload(L *lattice, int field)
+    {
+        const double *links[4];
+        for (int mu = 0; mu < 4; mu++) links[mu] = sites(lattice[mu]);
+        check(glacx_upload_field(m_ctx, field, links), "upload");
+    }
+
+    glacx_ctx *m_ctx = nullptr;
+    int m_dims[4];
+    unsigned long m_volume = 0;
+    std::vector<Binding> m_bind;
+};
+
+// ------------------------------------------------------------ 3x3 helpers ---
+// GLAC's SU3 storage: U_ij at mat[6i + 2j] (re), mat[6i + 2j + 1] (im).
+inline void mul(const double *a, const double *b, double *c)
+{
+    double t[18];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double re = 0, im = 0;
+            for (int k = 0; k < 3; k++) {
+                re += a[6 * i + 2 * k] * b[6 * k + 2 * j] - a[6 * i + 2 * k + 1] * b[6 * k + 2 * j + 1];
+                im += a[6 * i + 2 * k] * b[6 * k + 2 * j + 1] + a[6 * i + 2 * k + 1] * b[6 * k + 2 * j];
+            }
+            t[6 * i + 2 * j] = re;
+            t[6 * i + 2 * j + 1] = im;
+        }
+    for (int n = 0; n < 18; n++) c[n] = t[n];
+}
+inline void dag(const double *a, double *c)
+{
+    double t[18];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            t[6 * i + 2 * j] = a[6 * j + 2 * i];
+            t[6 * i + 2 * j + 1] = -a[6 * j + 2 * i + 1];
+        }
+    for (int n = 0; n < 18; n++) c[n] = t[n];
+}
+
+// ------------------------------------------------------------------ action ---
+// WilsonGaugeAction (GLAC/actions/action.h:49-68, wilsongaugeaction.cpp:6-86).
+// computeStaple / getDeltaAction keep GLAC's per-link FP64 host semantics (the
+// staple is cached between the two calls); getActionDerivative runs on the
+// GPU.  A GPU sweep replaces the per-link calls: see Metropolis below.
+template <class SU3, class L> class WilsonGaugeAction {
+public:
+    WilsonGaugeAction(double beta, Device *device) : m_beta(beta), m_dev(device) {}
+
+    double getDeltaAction(SU3 &U, SU3 &UPrime)
+    {
+        // -beta/3 Re tr[(U' - U) A]
+        double s = 0.0;
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++) {
+                const double dr = UPrime.mat[6 * i + 2 * k] - U.mat[6 * i + 2 * k];
+                const double di = UPrime.mat[6 * i + 2 * k + 1] - U.mat[6 * i + 2 * k + 1];
+                s += dr * m_staple[6 * k + 2 * i] - di * m_staple[6 * k + 2 * i + 1];
+            }
+        return -m_beta / 3.0 * s;
+    }
+
+    void computeStaple(L *lattice, int i, int j, int k, int l, int mu)
+    {
+        const std::vector<unsigned int> &N = lattice[0].m_dim;
+        const int c[4] = {i, j, k, l};
+        auto site = [&](int d1, int s1, int d2, int s2) {
+            int m[4] = {c[0], c[1], c[2], c[3]};
+            m[d1] = (m[d1] + s1 + (int)N[d1]) % (int)N[d1];
+            if (s2) m[d2] = (m[d2] + s2 + (int)N[d2]) % (int)N[d2];
+            return (unsigned long)m[0] + N[0] * ((unsigned long)m[1] + N[1] * ((unsigned long)m[2] + N[2] * m[3]));
+        };
+        auto link = [&](int dir, unsigned long s) { return lattice[dir].m_sites[s].mat; };
+        for (int n = 0; n < 18; n++) m_staple[n] = 0.0;
+        for (int nu = 0; nu < 4; nu++) {
+            if (nu == mu) continue;
+            double t1[18], t2[18], d[18];
+            // U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
+            dag(link(mu, site(nu, 1, mu, 0)), d);
+            mul(link(nu, site(mu, 1, nu, 0)), d, t1);
+            dag(link(nu, site(0, 0, 0, 0)), d);
+            mul(t1, d, t1);
+            // U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu)
+            dag(link(nu, site(mu, 1, nu, -1)), t2);
+            dag(link(mu, site(nu, -1, mu, 0)), d);
+            mul(t2, d, t2);
+            mul(t2, link(nu, site(nu, -1, mu, 0)), t2);
+            for (int n = 0; n < 18; n++) m_staple[n] += t1[n] + t2[n];
+        }
+    }
+
+    // Whole-lattice flow force Z_mu on the GPU.
+    L getActionDerivative(L *lattice, int mu)
+    {
+        const int f = m_dev->acquire(lattice);
+        L out(lattice[mu].m_dim);
+        m_dev->check(glacx_action_derivative(m_dev->ctx(), f, mu, reinterpret_cast<double *>(&out.m_sites[0])),
+                     "action_derivative");
+        return out;
+    }
+
+    const double *staple() const { return m_staple; }
+
+private:
+    double m_beta;
+    Device *m_dev;
+    double m_staple[18];
+};
+
+// -------------------------------------------------------------------- flow ---
+// Flow (GLAC/flow/flow.h:40-42): one RK3 step, in place.
+template <class L> class Flow {
+public:
+    Flow(Device *device, double epsilon) : m_dev(device), m_eps(epsilon) {}
+    void flowField(L *lattice)
+    {
+        const int f = m_dev->acquire(lattice);
+        m_dev->check(glacx_flow(m_dev->ctx(), f, 1, m_eps, nullptr, nullptr, nullptr, nullptr), "flow");
+        m_dev->markDeviceModified(lattice);
+    }
+
+private:
+    Device *m_dev;
+    double m_eps;
+};
+
+// ------------------------------------------------------------- observables ---
+// MasterSamplerTopcXYZ (GLAC/observables/mastersamplertopcxyz.cpp:236-384):
+// per observation index the plaquette, topological charge, E_raw and the
+// per-time-slice charge.  getObservablesVector / copyObservable keep GLAC's
+// ordering {plaq, topc, energy, topct[0..Nt-1]} (:227-244).
+template <class L> class MasterSamplerTopcXYZ {
+public:
+    MasterSamplerTopcXYZ(Device *device, unsigned int nObs)
+        : m_dev(device), m_nt(device->dims()[3]), m_obs(nObs, std::vector<double>(3 + device->dims()[3], 0.0))
+    {
+    }
+
+    void calculate(L *lattice, unsigned int iObs)
+    {
+        if (iObs >= m_obs.size()) fatal("observation index out of range");
+        const int f = m_dev->acquire(lattice);
+        std::vector<double> &o = m_obs[iObs];
+        m_dev->check(glacx_measure(m_dev->ctx(), f, &o[0], &o[2], &o[1], &o[3]), "measure");
+    }
+
+    std::vector<double> getObservablesVector(unsigned int iObs) const { return m_obs[iObs]; }
+    void copyObservable(unsigned int iObs, const std::vector<double> &obs) { m_obs[iObs] = obs; }
+    double getObservable(unsigned int iObs) const { return m_obs[iObs][0]; }
+    void reset()
+    {
+        for (auto &o : m_obs) std::fill(o.begin(), o.end(), 0.0);
+    }
+
+private:
+    Device *m_dev;
+    int m_nt;
+    std::vector<std::vector<double>> m_obs;
+};
+
+// --------------------------------------------------------------- Metropolis ---
+// System::update (GLAC/system.cpp:333-361) as one GPU checkerboard sweep.
+// The acceptance rate of each sweep is accumulated as GLAC's m_acceptanceScore
+// (:359-360).
+template <class L> class Metropolis {
+public:
+    Metropolis(Device *device, double beta, double eps, int nUpdates, unsigned long long seed, int reunitEvery = 1)
+        : m_dev(device), m_beta(beta), m_eps(eps), m_nhit(nUpdates), m_seed(seed), m_reunit(reunitEvery)
+    {
+    }
+    void update(L *lattice)
+    {
+        const int f = m_dev->acquire(lattice);
+        if (f != GLACX_FIELD_MAIN) fatal("the Markov chain must live in the main device field");
+        double acc = 0.0;
+        m_dev->check(glacx_metropolis(m_dev->ctx(), 1, m_beta, m_eps, m_nhit, m_reunit, m_seed, m_sweep, &acc, nullptr),
+                     "metropolis");
+        m_sweep++;
+        m_acceptanceScore += acc;
+        m_dev->markDeviceModified(lattice);
+    }
+    double acceptanceScore() const { return m_acceptanceScore; }
+    unsigned long long sweeps() const { return m_sweep; }
+
+private:
+    Device *m_dev;
+    double m_beta, m_eps;
+    int m_nhit;
+    unsigned long long m_seed, m_sweep = 0;
+    int m_reunit;
+    double m_acceptanceScore = 0.0;
+};
+
+} // namespace glac_gpu

@@ -53,7 +53,9 @@ enum glacx_kernel {
     GLACX_K_MEASURE = 1,
     GLACX_K_METROPOLIS = 2,
     GLACX_K_PLAQUETTE = 3,
-    GLACX_K_COUNT = 4
+    GLACX_K_METROPOLIS_LEARNED = 4,
+    GLACX_K_TRAIN = 5,
+    GLACX_K_COUNT = 6
 };
 
 /* Create a context for an Nx x Ny x Nz x Nt periodic lattice (all extents
@@ -123,6 +125,41 @@ int glacx_flow(glacx_ctx *ctx, int field, int n_steps, double eps,
  * (WilsonGaugeAction::getActionDerivative,
  * GLAC/actions/wilsongaugeaction.cpp:55-86). */
 int glacx_action_derivative(glacx_ctx *ctx, int field, int mu, double *z_out);
+
+/* ---- Learned proposal.  No counterpart in GLAC: it replaces the fixed-eps
+ * RST proposal (GLAC/math/matrices/su3matrixgenerator.h:170-321) in the
+ * Metropolis update (GLAC/system.cpp:321-361).  A network maps six gauge
+ * invariants of the staple to the Cholesky factor of the covariance of an
+ * su(3) step; parameters are 2468 floats packed W1[32][6], b1[32],
+ * W2[32][32], b2[32], W3[36][32], b3[36]; features are standardised with
+ * mean[6], std[6]. */
+
+/* Start a new training-sample buffer holding up to `capacity` samples. */
+int glacx_samples_reset(glacx_ctx *ctx, size_t capacity);
+/* Append (staple, link) pairs of every `stride`-th link of each direction
+ * and parity of GLACX_FIELD_MAIN; *count receives the total held. */
+int glacx_samples_collect(glacx_ctx *ctx, int stride, size_t *count);
+/* Feature mean and standard deviation over the held samples; also stored
+ * as the context's standardisation. */
+int glacx_samples_stats(glacx_ctx *ctx, double beta, float mean[6], float std[6]);
+/* Set / get the network parameters and standardisation. */
+int glacx_proposal_set(glacx_ctx *ctx, const float *params, const float mean[6], const float std[6]);
+int glacx_proposal_get(glacx_ctx *ctx, float *params, float mean[6], float std[6]);
+/* `steps` Adam steps (learning rate lr) on batches of `batch` samples with
+ * `draws` proposal draws each; loss[i] (may be NULL) receives each step's
+ * loss (the negative expected squared jump distance). */
+int glacx_train(glacx_ctx *ctx, int steps, int batch, int draws, double lr, double beta, uint64_t seed,
+                double *loss);
+/* Loss and gradient of the current parameters on n host samples (n x 36
+ * floats: staple then link, GLAC's 18-real layout each) with the given
+ * draws z (n x draws x 8 floats): the device training step without the
+ * update, for checking. */
+int glacx_train_grad(glacx_ctx *ctx, const float *samples, int n, const float *z, int draws, double beta,
+                     double *loss, float *grad);
+/* glacx_metropolis with the learned proposal (fixed-eps RST, step eps, for
+ * links whose standardised features fall outside [-8, 8]). */
+int glacx_metropolis_learned(glacx_ctx *ctx, int n_sweeps, double beta, double eps, int n_hit, int reunit_every,
+                             uint64_t seed, uint64_t sweep_offset, double *accept, double *plaq);
 
 /* Block until the context's stream is idle. */
 int glacx_synchronize(glacx_ctx *ctx);

@@ -76,6 +76,14 @@ def lib():
         L.or_cb_sweep.argtypes = [_dp, _i4, C.c_double, C.c_double, C.c_int, C.c_uint64,
                                   C.c_uint64, C.c_int]
         L.or_cb_sweep.restype = C.c_double
+        L.or_staple_features.argtypes = [_dp, C.c_double, _dp]
+        L.or_mlp_forward.argtypes = [_dp, _dp, _dp, _dp, _dp]
+        L.or_cholesky_from_output.argtypes = [_dp, _dp]
+        L.or_algebra_from_omega.argtypes = [_dp, _dp]
+        L.or_project_algebra.argtypes = [_dp, _dp]
+        L.or_learned_cb_sweep.argtypes = [_dp, _i4, C.c_double, C.c_double, C.c_int, C.c_uint64, C.c_uint64,
+                                          C.c_int, _dp, _dp, _dp]
+        L.or_learned_cb_sweep.restype = C.c_double
         _lib = L
     return _lib
 
@@ -275,3 +283,49 @@ def hot_start_haar(n, seed) -> np.ndarray:
 def cb_sweep(U, n, beta, eps, nhit, seed, sweep, reunit=True) -> float:
     assert U.flags["C_CONTIGUOUS"] and U.dtype == np.float64
     return lib().or_cb_sweep(U, dims4(n), beta, eps, nhit, seed, sweep, int(reunit))
+
+
+# ------------------------------------------------------------- learned proposal
+MLP_NPARAM = 2468
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def staple_features(A, beta) -> np.ndarray:
+    f = np.empty(6)
+    lib().or_staple_features(_f64(A), beta, f)
+    return f
+
+
+def mlp_forward(params, mean, std, f) -> np.ndarray:
+    o = np.empty(36)
+    lib().or_mlp_forward(_f64(params), _f64(mean), _f64(std), _f64(f), o)
+    return o
+
+
+def cholesky_from_output(o) -> np.ndarray:
+    L = np.empty(64)
+    lib().or_cholesky_from_output(_f64(o), L)
+    return L.reshape(8, 8)
+
+
+def algebra_from_omega(w) -> np.ndarray:
+    Y = np.empty(18)
+    lib().or_algebra_from_omega(_f64(w), Y)
+    return Y
+
+
+def project_algebra(N) -> np.ndarray:
+    g = np.empty(8)
+    lib().or_project_algebra(_f64(N), g)
+    return g
+
+
+def learned_cb_sweep(U, n, beta, nhit, seed, sweep, params, mean, std, reunit=True, eps=0.24) -> float:
+    """Checkerboard sweep with the learned proposal (links whose standardised
+    features leave [-8, 8] use GLAC's fixed-eps proposal)."""
+    assert U.flags["C_CONTIGUOUS"] and U.dtype == np.float64
+    return lib().or_learned_cb_sweep(U, dims4(n), beta, eps, nhit, seed, sweep, int(reunit), _f64(params),
+                                     _f64(mean), _f64(std))

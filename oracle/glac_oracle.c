@@ -811,3 +811,158 @@ double or_cb_sweep(double *U, const int n[4], double beta, double eps, int nhit,
             }
     return (double)accepted / (4.0 * nhit * (double)V);
 }
+
+/* ======================================================================== */
+/* Learned proposal (no GLAC counterpart)                                    */
+/* ======================================================================== */
+/*
+ * or_staple_features / or_mlp_forward / or_cholesky_from_output /
+ * or_algebra_from_omega / or_project_algebra / or_learned_cb_sweep.
+ * Pinned: features are gauge invariant (A -> g A h^dag); the algebra map is
+ * i w.T with T = lambda/2 (checked entry by entry against the Gell-Mann
+ * matrices) and projection is its adjoint (Re tr(i T_a Y(w)) = w_a / 2
+ * orthogonality); the sweep samples exp(beta/3 Re tr(U A)) (single-link
+ * histogram) and keeps GLAC's plaquette.
+ */
+
+static const int STREAM_LEARNED = 3;
+
+void or_staple_features(const double *A, double beta, double f[6])
+{
+    double Ad[18], B[18], B2[18], d[2];
+    or_su3_dag(A, Ad);
+    or_su3_mul(A, Ad, B);
+    or_su3_mul(B, B, B2);
+    or_su3_det(A, d);
+    f[0] = B[0] + B[8] + B[16];
+    f[1] = B2[0] + B2[8] + B2[16];
+    f[2] = d[0] * d[0] + d[1] * d[1];
+    f[3] = d[0];
+    f[4] = d[1];
+    f[5] = beta;
+}
+
+static inline double lrelu(double v) { return v > 0.0 ? v : 0.01 * v; }
+
+void or_mlp_forward(const double *P, const double *mean, const double *stdv, const double f[6], double o[36])
+{
+    const double *W1 = P, *b1 = W1 + 32 * 6, *W2 = b1 + 32, *b2 = W2 + 32 * 32, *W3 = b2 + 32, *b3 = W3 + 36 * 32;
+    double x[6], h1[32], h2[32];
+    for (int k = 0; k < 6; k++) x[k] = (f[k] - mean[k]) / stdv[k];
+    for (int i = 0; i < 32; i++) {
+        double s = b1[i];
+        for (int k = 0; k < 6; k++) s += W1[i * 6 + k] * x[k];
+        h1[i] = lrelu(s);
+    }
+    for (int i = 0; i < 32; i++) {
+        double s = b2[i];
+        for (int k = 0; k < 32; k++) s += W2[i * 32 + k] * h1[k];
+        h2[i] = lrelu(s);
+    }
+    for (int i = 0; i < 36; i++) {
+        double s = b3[i];
+        for (int k = 0; k < 32; k++) s += W3[i * 32 + k] * h2[k];
+        o[i] = s;
+    }
+}
+
+void or_cholesky_from_output(const double o[36], double L[64])
+{
+    for (int k = 0; k < 64; k++) L[k] = 0.0;
+    for (int a = 0; a < 8; a++) {
+        L[a * 8 + a] = exp(o[a]);
+        for (int b = 0; b < a; b++) L[a * 8 + b] = o[8 + a * (a - 1) / 2 + b];
+    }
+}
+
+/* Y = i sum_a w_a lambda_a / 2 as a 3x3 matrix in GLAC's layout. */
+void or_algebra_from_omega(const double w[8], double Y[18])
+{
+    const double r3 = 1.0 / sqrt(3.0);
+    su3_zero(Y);
+    IM(Y, 0, 0) = 0.5 * (w[2] + r3 * w[7]);
+    IM(Y, 1, 1) = 0.5 * (-w[2] + r3 * w[7]);
+    IM(Y, 2, 2) = -r3 * w[7];
+    /* (0,1): lambda_1, lambda_2 ; (0,2): lambda_4, lambda_5 ; (1,2): lambda_6, lambda_7 */
+    const int pi[3] = {0, 0, 1}, pj[3] = {1, 2, 2}, re[3] = {1, 4, 6}, im[3] = {0, 3, 5};
+    for (int k = 0; k < 3; k++) {
+        const int i = pi[k], j = pj[k];
+        RE(Y, i, j) = 0.5 * w[re[k]];
+        IM(Y, i, j) = 0.5 * w[im[k]];
+        RE(Y, j, i) = -0.5 * w[re[k]];
+        IM(Y, j, i) = 0.5 * w[im[k]];
+    }
+}
+
+/* g_a = Re tr(i lambda_a / 2 N) */
+void or_project_algebra(const double *N, double g[8])
+{
+    const double r3 = 1.0 / sqrt(3.0);
+    g[0] = -0.5 * (IM(N, 0, 1) + IM(N, 1, 0));
+    g[1] = 0.5 * (RE(N, 1, 0) - RE(N, 0, 1));
+    g[2] = -0.5 * (IM(N, 0, 0) - IM(N, 1, 1));
+    g[3] = -0.5 * (IM(N, 0, 2) + IM(N, 2, 0));
+    g[4] = 0.5 * (RE(N, 2, 0) - RE(N, 0, 2));
+    g[5] = -0.5 * (IM(N, 1, 2) + IM(N, 2, 1));
+    g[6] = 0.5 * (RE(N, 2, 1) - RE(N, 1, 2));
+    g[7] = -0.5 * r3 * (IM(N, 0, 0) + IM(N, 1, 1) - 2.0 * IM(N, 2, 2));
+}
+
+/* symmetric unit-variance draw: (c1 + c2) sqrt(6) with c uniform on the
+ * 2^24 midpoints of [-1/2, 1/2) -- exactly symmetric about 0 */
+static inline double stream_sym(uint32_t s[4])
+{
+    const double c1 = (double)(or_xoshiro128ss_next(s) >> 8) * 0x1p-24 - (0.5 - 0x1p-25);
+    const double c2 = (double)(or_xoshiro128ss_next(s) >> 8) * 0x1p-24 - (0.5 - 0x1p-25);
+    return (c1 + c2) * 2.449489742783178;
+}
+
+double or_learned_cb_sweep(double *U, const int n[4], double beta, double eps, int nhit, uint64_t seed, uint64_t sweep,
+                           int reunit, const double *params, const double *mean, const double *stdv)
+{
+    long V = volume(n);
+    unsigned long long accepted = 0;
+    for (int mu = 0; mu < 4; mu++)
+        for (int p = 0; p < 2; p++)
+            for (long s = 0; s < V; s++) {
+                int c[4];
+                decode(n, s, c);
+                if (((c[0] + c[1] + c[2] + c[3]) & 1) != p) continue;
+                double A[18], f[6], o[36], L[64], X[18], Y[18], Up[18];
+                double *Ul = U + ((long)mu * V + s) * 18;
+                staple_at(U, n, c, mu, A);
+                or_staple_features(A, beta, f);
+                /* staples unlike any in training fall back to the fixed-eps proposal */
+                int learned = 1;
+                for (int k = 0; k < 6; k++)
+                    if (fabs((f[k] - mean[k]) / stdv[k]) > 8.0) learned = 0;
+                or_mlp_forward(params, mean, stdv, f, o);
+                or_cholesky_from_output(o, L);
+                uint32_t st[4];
+                link_stream(st, seed, s, mu, STREAM_LEARNED, sweep);
+                for (int h = 0; h < nhit; h++) {
+                    if (learned) {
+                        double z[8], w[8];
+                        for (int a = 0; a < 8; a++) z[a] = stream_sym(st);
+                        for (int a = 0; a < 8; a++) {
+                            w[a] = 0.0;
+                            for (int b = 0; b <= a; b++) w[a] += L[a * 8 + b] * z[b];
+                        }
+                        or_algebra_from_omega(w, Y);
+                        or_su3_exp(Y, X);
+                    } else {
+                        double u[13];
+                        for (int k = 0; k < 13; k++) u[k] = stream_centered(st);
+                        or_generate_rst(u, eps, X);
+                    }
+                    or_su3_mul(X, Ul, Up);
+                    double dS = or_delta_action(Ul, Up, A, beta);
+                    if (exp(-dS) > stream_open(st)) {
+                        memcpy(Ul, Up, sizeof Up);
+                        accepted++;
+                    }
+                }
+                if (reunit) or_reunitarize(Ul);
+            }
+    return (double)accepted / (4.0 * nhit * (double)V);
+}

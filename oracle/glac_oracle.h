@@ -74,7 +74,25 @@ void   or_hot_start_haar(double *U, const int n[4], uint64_t seed);
 double or_cb_sweep(double *U, const int n[4], double beta, double eps, int nhit,
                    uint64_t seed, uint64_t sweep, int reunit);
 
+/* ---- learned proposal (no GLAC counterpart) ------------------------------
+ * Parameter packing (float64 here, float32 on the device): W1[32][6], b1[32],
+ * W2[32][32], b2[32], W3[36][32], b3[36]  (OR_MLP_NPARAM values).
+ * Features of a staple A (gauge invariants): tr AA^dag, tr (AA^dag)^2,
+ * |det A|^2, Re det A, Im det A, beta; standardised with mean[6], std[6].
+ * Output o[36] -> lower-triangular L: L_aa = exp(o_a), L_ab = o[8 + a(a-1)/2 + b].
+ * Proposal X = exp(i sum_a w_a lambda_a / 2), w = L z. */
+#define OR_MLP_NPARAM 2468
+void   or_staple_features(const double *A, double beta, double f[6]);
+void   or_mlp_forward(const double *params, const double *mean, const double *stdv, const double f[6],
+                      double o[36]);
+void   or_cholesky_from_output(const double o[36], double L[64]);
+void   or_algebra_from_omega(const double w[8], double Y[18]);
+void   or_project_algebra(const double *N, double g[8]);
+double or_learned_cb_sweep(double *U, const int n[4], double beta, double eps, int nhit, uint64_t seed,
+                           uint64_t sweep, int reunit, const double *params, const double *mean, const double *stdv);
+
 #ifdef __cplusplus
 }
 #endif
 #endif
+

@@ -1,0 +1,165 @@
+// Exercises glac_amd/host/glac_gpu.hpp the way GLAC's own code would, with a
+// stand-in of GLAC's public Lattice<SU3> shape (GLAC/math/lattice.h:23-29).
+//
+//   host_adapter_check cpu   GLAC's dS = -36 test on a unit field
+//                            (GLAC/tests/unit_tests/actiontests.cpp:67-114)
+//   host_adapter_check gpu   flowConfiguration-style loop (calculate, then
+//                            flowField + calculate per step) and Metropolis
+//                            updates through the adapter, checked against
+//                            the FP64 oracle; device residency is checked by
+//                            counting host<->device transfers.
+// Prints "PASS ..." lines; exits non-zero on the first failure.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "../glac_amd/host/glac_gpu.hpp"
+#include "../oracle/glac_oracle.h"
+
+struct SU3 {
+    double mat[18];
+};
+
+template <class T> struct Lattice {
+    std::vector<T> m_sites;
+    std::vector<unsigned int> m_dim;
+    unsigned long m_latticeSize = 0;
+    Lattice() {}
+    explicit Lattice(const std::vector<unsigned int> &dim) { allocate(dim); }
+    void allocate(const std::vector<unsigned int> &dim)
+    {
+        m_dim = dim;
+        m_latticeSize = (unsigned long)dim[0] * dim[1] * dim[2] * dim[3];
+        m_sites.assign(m_latticeSize, T());
+    }
+};
+
+#define REQUIRE(cond, ...)                                                                       \
+    do {                                                                                          \
+        if (!(cond)) {                                                                            \
+            std::printf("FAIL line %d: ", __LINE__);                                             \
+            std::printf(__VA_ARGS__);                                                             \
+            std::printf("\n");                                                                    \
+            return 1;                                                                             \
+        }                                                                                         \
+    } while (0)
+
+static void set_unit(Lattice<SU3> *L)
+{
+    for (int mu = 0; mu < 4; mu++)
+        for (auto &s : L[mu].m_sites) {
+            for (double &v : s.mat) v = 0.0;
+            s.mat[0] = s.mat[8] = s.mat[16] = 1.0;
+        }
+}
+
+static int run_cpu()
+{
+    const std::vector<unsigned int> N = {4, 4, 4, 4};
+    Lattice<SU3> L[4];
+    for (int mu = 0; mu < 4; mu++) L[mu].allocate(N);
+    set_unit(L);
+    glac_gpu::WilsonGaugeAction<SU3, Lattice<SU3>> S(6.0, nullptr);
+    SU3 two{};
+    two.mat[0] = two.mat[8] = two.mat[16] = 2.0;
+    for (unsigned x = 0; x < 4; x++)
+        for (unsigned t = 0; t < 4; t++)
+            for (int mu = 0; mu < 4; mu++) {
+                S.computeStaple(L, x, 1, 2, t, mu);
+                const unsigned long site = x + 4 * (1 + 4 * (2 + 4 * t));
+                const double dS = S.getDeltaAction(L[mu].m_sites[site], two);
+                REQUIRE(std::fabs(dS + 36.0) <= 1e-15, "dS = %.17g", dS);
+            }
+    std::printf("PASS dS = -36 on the unit field\n");
+    return 0;
+}
+
+static int run_gpu()
+{
+    const std::vector<unsigned int> N = {4, 4, 4, 8};
+    const int n[4] = {4, 4, 4, 8};
+    const unsigned long V = 4 * 4 * 4 * 8;
+    Lattice<SU3> L[4];
+    for (int mu = 0; mu < 4; mu++) L[mu].allocate(N);
+    // oracle Haar field in GLAC's layout
+    std::vector<double> U(4 * V * 18);
+    or_hot_start_haar(U.data(), n, 77);
+    for (int mu = 0; mu < 4; mu++)
+        for (unsigned long s = 0; s < V; s++)
+            for (int k = 0; k < 18; k++) L[mu].m_sites[s].mat[k] = U[(mu * V + s) * 18 + k];
+
+    glac_gpu::Device dev(N, GLACX_FP64, 0);
+    glac_gpu::WilsonGaugeAction<SU3, Lattice<SU3>> S(6.0, &dev);
+    glac_gpu::Flow<Lattice<SU3>> flow(&dev, 0.02);
+    glac_gpu::MasterSamplerTopcXYZ<Lattice<SU3>> obs(&dev, 6);
+
+    // getActionDerivative vs the oracle
+    for (int mu = 0; mu < 4; mu++) {
+        Lattice<SU3> Z = S.getActionDerivative(L, mu);
+        std::vector<double> Zo(V * 18);
+        or_action_derivative(U.data(), n, mu, Zo.data());
+        double err = 0;
+        for (unsigned long s = 0; s < V; s++)
+            for (int k = 0; k < 18; k++) err = std::fmax(err, std::fabs(Z.m_sites[s].mat[k] - Zo[s * 18 + k]));
+        REQUIRE(err < 1e-13, "getActionDerivative mu %d err %g", mu, err);
+    }
+    std::printf("PASS getActionDerivative\n");
+
+    // flowConfiguration loop: calculate(0); {flowField; calculate(i+1)} x 5
+    obs.calculate(L, 0);
+    for (int i = 0; i < 5; i++) {
+        flow.flowField(L);
+        obs.calculate(L, i + 1);
+    }
+    std::vector<double> W = U, qt(8);
+    double out[3];
+    for (int i = 0; i <= 5; i++) {
+        if (i > 0) or_flow_step(W.data(), n, 0.02);
+        or_measure(W.data(), n, out, qt.data());
+        const std::vector<double> o = obs.getObservablesVector(i);
+        REQUIRE(std::fabs(o[0] - out[0]) < 1e-12, "plaq step %d", i);
+        REQUIRE(std::fabs(o[1] - out[2]) < 1e-12, "topc step %d", i);
+        REQUIRE(std::fabs(o[2] / out[1] - 1.0) < 1e-11, "energy step %d", i);
+        for (int t = 0; t < 8; t++) REQUIRE(std::fabs(o[3 + t] - qt[t]) < 1e-12, "topct step %d t %d", i, t);
+    }
+    // the host copy is stale until syncHost
+    REQUIRE(std::fabs(L[0].m_sites[0].mat[0] - U[0]) == 0.0, "host copy changed without syncHost");
+    dev.syncHost(L);
+    double err = 0;
+    for (int mu = 0; mu < 4; mu++)
+        for (unsigned long s = 0; s < V; s++)
+            for (int k = 0; k < 18; k++) err = std::fmax(err, std::fabs(L[mu].m_sites[s].mat[k] - W[(mu * V + s) * 18 + k]));
+    REQUIRE(err < 1e-12, "flowed field after syncHost err %g", err);
+    std::printf("PASS flow + clover observables through Flow / MasterSamplerTopcXYZ\n");
+
+    // Metropolis through System::update's interface on a separate chain
+    Lattice<SU3> M[4];
+    for (int mu = 0; mu < 4; mu++) M[mu].allocate(N);
+    set_unit(M);
+    glac_gpu::Device dev2(N, GLACX_FP64, 0);
+    glac_gpu::Metropolis<Lattice<SU3>> metro(&dev2, 6.0, 0.24, 10, 2024);
+    for (int s = 0; s < 20; s++) metro.update(M);
+    dev2.syncHost(M);
+    const double acc = metro.acceptanceScore() / 20.0;
+    REQUIRE(acc > 0.05 && acc < 0.5, "acceptance %g", acc);
+    double dmax = 0;
+    for (int mu = 0; mu < 4; mu++)
+        for (auto &s : M[mu].m_sites) {
+            double d[2];
+            or_su3_det(s.mat, d);
+            dmax = std::fmax(dmax, std::fabs(d[0] - 1.0) + std::fabs(d[1]));
+        }
+    REQUIRE(dmax < 1e-12, "det deviation %g", dmax);
+    std::printf("PASS Metropolis update, acceptance %.4f\n", acc);
+    return 0;
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 2) return 2;
+    const std::string mode = argv[1];
+    if (mode == "cpu") return run_cpu();
+    if (mode == "gpu") return run_gpu();
+    return 2;
+}

@@ -1,0 +1,118 @@
+"""GPU parity of the learned proposal (no GLAC counterpart): the device
+training step against the FP64 numpy reference (oracle/training.py), the
+learned-proposal sweep against the oracle's on the same streams, and the
+end-to-end pipeline (collect -> standardise -> train -> sample) against the
+plain sampler's plaquette."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import oracle.training as T
+from fieldtools import random_su3
+
+pytestmark = pytest.mark.gpu
+glacx = pytest.importorskip("glac_amd")
+
+
+def banner(msg):
+    print(f"\n[gpu case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def bf16_round(p):
+    x = np.ascontiguousarray(p, dtype=np.float32)
+    u = x.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return r.view(np.float32)
+
+
+def make_samples(rng, S):
+    A = np.array([sum(random_su3(rng) for _ in range(6)) for _ in range(S)])
+    U = random_su3(rng, (S,))
+    smp = np.concatenate([O.to_pairs(A), O.to_pairs(U)], axis=1)
+    return A, U, smp
+
+
+def test_train_gradient_matches_reference():
+    banner("learned: training loss/gradient vs numpy FP64")
+    rng = np.random.default_rng(10)
+    S, K, beta = 300, 3, 6.0
+    A, U, smp = make_samples(rng, S)
+    f = T.features(A, beta)
+    mean, std = f.mean(0).astype(np.float32), (f.std(0) + 1e-3).astype(np.float32)
+    p = T.init_params(rng, scale0=0.3)
+    p[T.SLICES["W3"]] += rng.normal(size=36 * 32) * 0.05
+    p = p.astype(np.float32)
+    z = rng.normal(size=(S, K, 8)).astype(np.float32)
+    with glacx.Lattice((4, 4, 4, 4), glacx.FP32_LINKS) as L:
+        L.proposal_set(p, mean, std)
+        loss, g = L.train_grad(smp.astype(np.float32), z, beta)
+    want_loss, want_g = T.loss_and_grad(p.astype(np.float64), mean.astype(np.float64), std.astype(np.float64),
+                                        A.astype(np.complex64).astype(np.complex128),
+                                        U.astype(np.complex64).astype(np.complex128), beta, z.astype(np.float64))
+    assert abs(loss - want_loss) < 1e-4 * abs(want_loss)
+    scale = np.abs(want_g).max()
+    assert np.abs(g - want_g).max() < 2e-3 * scale, np.abs(g - want_g).max() / scale
+
+
+@pytest.mark.parametrize("prec", [glacx.FP64, glacx.FP32_LINKS], ids=["fp64", "fp32"])
+def test_learned_sweep_follows_oracle_streams(prec):
+    n = (4, 4, 4, 8)
+    banner(f"learned sweep vs oracle {n} prec={prec}")
+    rng = np.random.default_rng(11)
+    p = T.init_params(rng, scale0=0.3)
+    p[T.SLICES["W3"]] += rng.normal(size=36 * 32) * 0.02
+    pb = bf16_round(p)
+    U = O.hot_start_haar(n, 3)
+    for s in range(3):
+        O.cb_sweep(U, n, 6.0, 0.24, 10, 3, s)
+    # standardisation from the oracle field's staples
+    f = np.array([O.staple_features(O.staple(U, n, x, 1, 2, t, mu), 6.0)
+                  for x in range(4) for t in range(8) for mu in range(4)])
+    mean, std = f.mean(0).astype(np.float32), (f.std(0) + 1e-3).astype(np.float32)
+    W = U.copy()
+    acc = O.learned_cb_sweep(W, n, 6.0, 10, 21, 0, pb.astype(np.float64), mean.astype(np.float64),
+                             std.astype(np.float64))
+    with glacx.Lattice(n, prec) as L:
+        L.upload(U)
+        L.proposal_set(p, mean, std)
+        got_acc = L.metropolis_learned(1, 6.0, 0.24, 10, reunit_every=1, seed=21)
+        got = L.download()
+    nlinks = 4 * 10 * np.prod(n)
+    # the network runs in FP32 on the device (FP64 in the oracle): allow a
+    # handful of flipped near-threshold decisions
+    assert abs(acc - got_acc[0]) * nlinks <= 3
+    close = np.abs(got - W).reshape(4, -1, 18).max(axis=-1) < (1e-5 if prec == glacx.FP64 else 1e-4)
+    assert close.mean() > 0.99
+
+
+def test_pipeline_trains_and_samples_the_right_distribution():
+    n = (8, 8, 8, 8)
+    beta = 6.0
+    banner(f"learned pipeline {n}")
+    with glacx.Lattice(n, glacx.FP32_LINKS) as L:
+        L.cold_start()
+        L.metropolis(150, beta, 0.24, 10, seed=5)
+        L.samples_reset(100000)
+        for i in range(10):
+            L.metropolis(5, beta, 0.24, 10, seed=5, sweep_offset=150 + 5 * i)
+            count = L.samples_collect(stride=4)
+        assert count == 10 * 8 * (np.prod(n) // 2 // 4)
+        mean, std = L.samples_stats(beta)
+        assert np.all(std > 0)
+        p0 = T.init_params(np.random.default_rng(12), scale0=0.15).astype(np.float32)
+        L.proposal_set(p0, mean, std)
+        loss = L.train(300, batch=2048, draws=4, lr=2e-3, beta=beta, seed=9)
+        assert np.all(np.isfinite(loss))
+        assert loss[-50:].mean() < 0.8 * loss[:20].mean()   # ESJD (= -loss) grew
+        p1, m1, s1 = L.proposal_get()
+        assert np.all(np.isfinite(p1)) and np.allclose(m1, mean)
+        # sampling: same equilibrium plaquette as the plain chain
+        acc_plain, pl_plain = L.metropolis(200, beta, 0.24, 10, seed=6, sweep_offset=1000, plaquette=True)
+        acc_l, pl_l = L.metropolis_learned(200, beta, 0.24, 10, seed=7, plaquette=True)
+    from glac_amd.analysis import tau_int
+    a, b = tau_int(pl_plain[50:]), tau_int(pl_l[50:])
+    assert abs(a["mean"] - b["mean"]) < 4 * np.hypot(a["err"], b["err"]) + 1e-4, (a, b)
+    assert 0.05 < acc_l.mean() < 0.95
