@@ -17,9 +17,11 @@
 // (8 planes of T per direction).
 //
 // Launch: 256-thread blocks = 64 sites x 4 directions; wavefront w handles
-// mu = w for the block's 64 consecutive sites (coalesced plane accesses), so
-// the four links of a site -- which share most staple links -- are processed
-// on the same CU at the same time.
+// mu = w for the block's 64 sites, the h-th to (h+31)-th of each parity (two
+// contiguous runs of plane entries, side by side in the lattice).  Even and
+// odd sites are neighbours of each other, so a block reads the links around
+// one spot of both parity halves, and the four links of a site -- which
+// share most staple links -- are processed on the same CU at the same time.
 #include "expsu3.hpp"
 #include "kernels.hpp"
 
@@ -31,8 +33,10 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
                                                             T *__restrict__ acc, Geom g, T cz, T ca)
 {
     const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // one direction per wavefront
-    const int idx = xcd_block(blockIdx.x, gridDim.x) * kFlowSites + (threadIdx.x & 63);
-    if (idx >= g.v) return;
+    const int lane = threadIdx.x & 63;
+    const int h = xcd_block(blockIdx.x, gridDim.x) * (kFlowSites / 2) + (lane & 31);
+    if (h >= g.vh) return;
+    const int idx = parity_index(g, lane >> 5, h);
     const Coord x = decode(g, idx);
     const M3<T> a = staple<T>(win, g, x, mu);
     const M3<T> u = load_link<T>(win, g.v, mu, idx);
@@ -73,7 +77,7 @@ template <class T>
 void launch_flow_stage(const void *win, void *wout, void *acc, const Geom &g, int stage, double eps, hipStream_t s)
 {
     using V2 = typename vec2<T>::type;
-    const unsigned blocks = (unsigned)((g.v + kFlowSites - 1) / kFlowSites);
+    const unsigned blocks = (unsigned)((g.vh + kFlowSites / 2 - 1) / (kFlowSites / 2));
     const V2 *in = (const V2 *)win;
     V2 *out = (V2 *)wout;
     T *a = (T *)acc;

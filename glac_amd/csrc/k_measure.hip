@@ -55,9 +55,11 @@ __device__ __forceinline__ M3<T> clover(const typename vec2<T>::type *__restrict
 // Launch shape: 192-thread blocks = 64 sites x 3 wavefronts; wavefront w
 // evaluates the clover pair nu = w + 1 (planes (0, nu) and (rho, sigma)) for
 // the block's 64 sites, so the three pairs of a site run side by side.
-// Block b covers chunk c = b / Nt of time slice t = b % Nt: consecutive
-// blocks walk through t at fixed spatial position, so the t +- 1 links a
-// clover needs were just loaded by the neighbouring blocks.
+// Blocks follow the storage order: block b takes chunk jc = b % cpr of the
+// run (blk, lt), i.e. its sites jc * 32 .. jc * 32 + 31 of both parities
+// (neighbours of each other), all on time slice t = (blk / nzb) bt + lt.
+// Its partials go to slot (c, t), c = zb cpr + jc being the chunk's position
+// within its time slice, where the per-slice sum finds them in a fixed order.
 constexpr int kMeasSites = 64;
 constexpr int kMeasWaves = 3;
 
@@ -66,12 +68,16 @@ __global__ __launch_bounds__(kMeasSites *kMeasWaves) void measure_kernel(const t
                                                                          Geom g, int cps, double *__restrict__ partials)
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int chunk = b / g.n[3], t = b - chunk * g.n[3];
-    const int r = chunk * kMeasSites + (threadIdx.x & 63);
+    const int rest = fdiv(b, g.d_cpr), jc = b - rest * g.cpr;
+    const int lt = rest & ((1 << g.lbt) - 1), blk = rest >> g.lbt;
+    const int bt = fdiv(blk, g.d_nzb), zb = blk - bt * g.nzb;
+    const int t = (bt << g.lbt) + lt, c = zb * g.cpr + jc;
+    const int lane = threadIdx.x & 63;
+    const int k = jc * (kMeasSites / 2) + (lane & 31);
     const int nu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) + 1;  // one clover pair per wavefront
     double acc[3] = {0.0, 0.0, 0.0};
-    if (r < g.vs) {
-        const Coord x = decode(g, slice_index(g, t, r));
+    if (k < g.run) {
+        const Coord x = decode(g, run_index(g, blk, lane >> 5, lt, k));
         const int rho = nu % 3 + 1, sigma = rho % 3 + 1;
         T p1, p2;
         const M3<T> c1 = traceless_antiherm2(clover<T>(f, g, x, 0, nu, p1));
@@ -82,9 +88,10 @@ __global__ __launch_bounds__(kMeasSites *kMeasWaves) void measure_kernel(const t
     }
     block_sum<3, kMeasWaves>(acc);
     if (threadIdx.x == 0) {
-        partials[3 * b + 0] = acc[0];
-        partials[3 * b + 1] = acc[1];
-        partials[3 * b + 2] = acc[2];
+        double *q = partials + 3 * (c * g.n[3] + t);
+        q[0] = acc[0];
+        q[1] = acc[1];
+        q[2] = acc[2];
     }
 }
 
@@ -169,7 +176,8 @@ __global__ __launch_bounds__(kBlock) void plaquette_finish_kernel(const double *
     if (threadIdx.x == 0) out[0] = acc[0] / (18.0 * (double)g.v);
 }
 
-static inline int chunks_per_slab(const Geom &g) { return (g.vs + kMeasSites - 1) / kMeasSites; }
+// 64-site chunks per time slice: nzb runs of each slice, cpr chunks per run
+static inline int chunks_per_slab(const Geom &g) { return g.nzb * g.cpr; }
 
 // block partials followed by the per-slice sums
 int measure_partials_count(const Geom &g) { return 3 * g.n[3] * chunks_per_slab(g) + 3 * g.n[3]; }

@@ -1,24 +1,25 @@
 // lattice.hpp -- device layout of the gauge field and periodic neighbour
 // arithmetic.
 //
-// Sites are stored in 4D tiles.  Each extent n_k is split into tiles of
-// b_k = 4 sites (2 when n_k is not a multiple of 4; every extent is even), so
-// a tile holds b0 b1 b2 b3 sites (a power of two).  Tiles are ordered
-// lexicographically (x fastest) and each tile stores its even sites, then its
-// odd sites (parity p = (x + y + z + t) & 1), each half in intra-tile
-// lexicographic order halved:
-//     idx = tile * TV + p * TV / 2 + (intra_lex >> 1),
-//     tile = tx + T0 (ty + T1 (tz + T2 tt)),  intra_lex = lx + b0 (ly + b1 (lz + b2 lt)).
-// Since b0 is even, intra_lex >> 1 is unique per parity.
+// Sites are stored in slabs: the z and t extents are cut into blocks of
+// bz, bt = 4 sites (2 when the extent is not a multiple of 4; every extent is
+// even), a block being bz bt full x-y planes.  Blocks are ordered z-block
+// fastest; each block stores its even sites, then its odd sites, each half in
+// the order (lt, lz, y, x / 2):
+//     idx = (blk * 2 + p) * BH + ((lt * bz + lz) * Ny + y) * (Nx / 2) + (x >> 1),
+//     blk = z / bz + (Nz / bz) (t / bt),  lz = z % bz,  lt = t % bt,
+//     BH = bt bz Nx Ny / 2,  p = (x + y + z + t) & 1.
+// Since Nx is even, x >> 1 is unique per parity within an x-row.
 //
-// Why tiles: a 4D stencil touches the 8 nearest neighbours of every site.
-// With rows of full x-extent the t-neighbour of a site lies a whole 3D slice
-// away (9.4 MB of FP32 links at 32^4, more than an XCD's 4 MB L2), so every
-// link is fetched from beyond L2 several times per sweep.  In a 4^4 tile most
-// neighbours are in the same tile; only the tile faces (a quarter of the
-// sites per direction) reach into the next tile.  Sites of one parity within
-// a tile are contiguous, so a checkerboard sub-sweep and a 64-lane wavefront
-// still read contiguous runs.
+// Why this order: a wavefront of 64 lanes reads 64 consecutive entries of one
+// plane.  For every neighbour direction the lanes' neighbours are then one
+// or two contiguous runs (a step in y, z or t is a translation of the whole
+// run; a step in x moves within the same rows), so each load touches close
+// to the minimum number of cache lines: the texture address/data path, not
+// HBM, bounds these kernels when a load is scattered over many lines.  The
+// blocking in z and t keeps the t-neighbour of most sites (3 of 4) within
+// bz Nx Ny / 2 entries, a few hundred KB of links at 32^4, inside an XCD's L2,
+// where plain rows would put it a whole 3D slice away.
 //
 // A field is 36 complex planes (mu-major, then matrix entry row-major), each
 // V entries long: entry (i, j) of U_mu at idx lives at
@@ -37,51 +38,65 @@
 
 namespace glacx {
 
-// Division by a small runtime divisor d < 2^10 for dividends n < 2^30:
-// q = (n * m) >> 40 with m = floor(2^40 / d) + 1 is exact because
-// n * (m - 2^40 / d) / 2^40 < n / 2^40 < 1 / d.
+// Division by a runtime divisor 1 <= d < 2^30 for dividends 0 <= n < 2^30:
+// q = (n * m) >> s with s = 30 + ceil(log2 d) and m = floor(2^s / d) + 1.
+// With m = 2^s / d + e, 0 < e <= 1, n m / 2^s = n / d + n e / 2^s and
+// n e / 2^s < 2^30 / 2^s <= 1 / d, which never reaches the next integer
+// because the fractional part of n / d is at most (d - 1) / d.  n m < 2^62.
 struct FastDiv {
     uint64_t m;
+    int s;
     int d;
 };
 
 __host__ __device__ inline FastDiv make_fastdiv(int d)
 {
     FastDiv f;
+    int c = 0;
+    while ((1 << c) < d) c++;
     f.d = d;
-    f.m = ((uint64_t)1 << 40) / (uint64_t)d + 1;
+    f.s = 30 + c;
+    f.m = ((uint64_t)1 << f.s) / (uint64_t)d + 1;
     return f;
 }
 
-__host__ __device__ inline int fdiv(int n, const FastDiv &f) { return (int)(((uint64_t)n * f.m) >> 40); }
+__host__ __device__ inline int fdiv(int n, const FastDiv &f) { return (int)(((uint64_t)n * f.m) >> f.s); }
 
 struct Geom {
     int n[4];       // Nx, Ny, Nz, Nt
-    int lb[4];      // log2 of the tile extents b_k (1 or 2)
-    int tn[4];      // tile grid extents n_k / b_k
-    int ltv;        // log2 of the tile volume TV
+    int nxh;        // Nx / 2
+    int lbz, lbt;   // log2 of the block extents bz, bt (1 or 2)
+    int nzb;        // z-blocks, Nz / bz
+    int run;        // entries of one (block, parity, lt): bz Nx Ny / 2
+    int bh;         // entries of one (block, parity): bt run
     int v;          // V
     int vh;         // V / 2
     int vs;         // sites per time slice, Nx Ny Nz
-    FastDiv d_t0, d_t1, d_t2;
+    int cpr;        // 32-site chunks per run, ceil(run / 32)
+    FastDiv d_nxh, d_ny, d_nzb, d_run, d_bh, d_2bh, d_cpr;
 };
 
 __host__ inline Geom make_geom(const int dims[4])
 {
     Geom g;
-    g.ltv = 0;
-    for (int k = 0; k < 4; k++) {
-        g.n[k] = dims[k];
-        g.lb[k] = (dims[k] % 4 == 0) ? 2 : 1;
-        g.tn[k] = dims[k] >> g.lb[k];
-        g.ltv += g.lb[k];
-    }
+    for (int k = 0; k < 4; k++) g.n[k] = dims[k];
+    g.nxh = dims[0] / 2;
+    g.lbz = (dims[2] % 4 == 0) ? 2 : 1;
+    g.lbt = (dims[3] % 4 == 0) ? 2 : 1;
+    g.nzb = dims[2] >> g.lbz;
+    g.run = (1 << g.lbz) * dims[1] * g.nxh;
+    g.bh = g.run << g.lbt;
     g.v = dims[0] * dims[1] * dims[2] * dims[3];
     g.vh = g.v / 2;
     g.vs = dims[0] * dims[1] * dims[2];
-    g.d_t0 = make_fastdiv(g.tn[0]);
-    g.d_t1 = make_fastdiv(g.tn[1]);
-    g.d_t2 = make_fastdiv(g.tn[2]);
+    g.cpr = (g.run + 31) / 32;
+    g.d_cpr = make_fastdiv(g.cpr);
+    g.d_nxh = make_fastdiv(g.nxh);
+    g.d_ny = make_fastdiv(dims[1]);
+    g.d_nzb = make_fastdiv(g.nzb);
+    g.d_run = make_fastdiv(g.run);
+    g.d_bh = make_fastdiv(g.bh);
+    g.d_2bh = make_fastdiv(2 * g.bh);
     return g;
 }
 
@@ -105,29 +120,23 @@ struct Coord {
 // storage idx -> coordinates
 __host__ __device__ inline Coord decode(const Geom &g, int idx)
 {
-    const int tile = idx >> g.ltv;
-    const int within = idx & ((1 << g.ltv) - 1);
-    const int p = within >> (g.ltv - 1);
-    int j = within & ((1 << (g.ltv - 1)) - 1);
-    // j = hx + (b0 / 2) (ly + b1 (lz + b2 lt))
-    const int hx = j & ((1 << (g.lb[0] - 1)) - 1);
-    j >>= g.lb[0] - 1;
-    const int ly = j & ((1 << g.lb[1]) - 1);
-    j >>= g.lb[1];
-    const int lz = j & ((1 << g.lb[2]) - 1);
-    const int lt = j >> g.lb[2];
-    const int r0 = fdiv(tile, g.d_t0);
-    const int tx = tile - r0 * g.tn[0];
-    const int r1 = fdiv(r0, g.d_t1);
-    const int ty = r0 - r1 * g.tn[1];
-    const int tt = fdiv(r1, g.d_t2);
-    const int tz = r1 - tt * g.tn[2];
+    const int blk = fdiv(idx, g.d_2bh);
+    int j = idx - blk * 2 * g.bh;
+    const int p = j >= g.bh;
+    j -= p * g.bh;
+    const int q1 = fdiv(j, g.d_nxh);
+    const int hx = j - q1 * g.nxh;
+    const int q2 = fdiv(q1, g.d_ny);
+    const int y = q1 - q2 * g.n[1];
+    const int lz = q2 & ((1 << g.lbz) - 1);
+    const int lt = q2 >> g.lbz;
+    const int bt = fdiv(blk, g.d_nzb);
+    const int bz = blk - bt * g.nzb;
     Coord r;
-    r.c[1] = (ty << g.lb[1]) + ly;
-    r.c[2] = (tz << g.lb[2]) + lz;
-    r.c[3] = (tt << g.lb[3]) + lt;
-    // tile origins are even, so the parity of the site is that of its offsets
-    r.c[0] = (tx << g.lb[0]) + 2 * hx + ((ly + lz + lt + p) & 1);
+    r.c[1] = y;
+    r.c[2] = (bz << g.lbz) + lz;
+    r.c[3] = (bt << g.lbt) + lt;
+    r.c[0] = 2 * hx + ((y + r.c[2] + r.c[3] + p) & 1);
     return r;
 }
 
@@ -139,35 +148,35 @@ __host__ __device__ inline int lex_of(const Geom &g, const Coord &r)
 __host__ __device__ inline int index_of(const Geom &g, const Coord &r)
 {
     const int p = (r.c[0] + r.c[1] + r.c[2] + r.c[3]) & 1;
-    const int tile = (r.c[0] >> g.lb[0]) +
-                     g.tn[0] * ((r.c[1] >> g.lb[1]) + g.tn[1] * ((r.c[2] >> g.lb[2]) + g.tn[2] * (r.c[3] >> g.lb[3])));
-    const int intra = (r.c[0] & ((1 << g.lb[0]) - 1)) +
-                      ((r.c[1] & ((1 << g.lb[1]) - 1)) +
-                       ((r.c[2] & ((1 << g.lb[2]) - 1)) + ((r.c[3] & ((1 << g.lb[3]) - 1)) << g.lb[2]) << g.lb[1])
-                       << g.lb[0]);
-    return (tile << g.ltv) + (p << (g.ltv - 1)) + (intra >> 1);
+    const int blk = (r.c[2] >> g.lbz) + g.nzb * (r.c[3] >> g.lbt);
+    const int l = ((r.c[3] & ((1 << g.lbt) - 1)) << g.lbz) + (r.c[2] & ((1 << g.lbz) - 1));
+    return (blk * 2 + p) * g.bh + (l * g.n[1] + r.c[1]) * g.nxh + (r.c[0] >> 1);
 }
 
-// storage idx of the h-th site of parity p (h < V / 2): tiles in order, each
-// contributing its TV / 2 sites of that parity.
+// storage idx of the h-th site of parity p (h < V / 2): blocks in order, each
+// contributing its BH sites of that parity.
 __host__ __device__ inline int parity_index(const Geom &g, int p, int h)
 {
-    const int tile = h >> (g.ltv - 1);
-    return (tile << g.ltv) + (p << (g.ltv - 1)) + (h & ((1 << (g.ltv - 1)) - 1));
+    const int blk = fdiv(h, g.d_bh);
+    return h + (blk + p) * g.bh;
 }
 
-// storage idx of the r-th site of time slice t (r < Nx Ny Nz): the tiles of
-// tile row tt = t / b3 in order, each contributing its b0 b1 b2 sites at
-// local time lt = t % b3 (even then odd parity, each a contiguous run).
+// storage idx of the k-th site (k < run) of parity p in the run (blk, lt):
+// all its sites share t = (blk / nzb) bt + lt.
+__host__ __device__ inline int run_index(const Geom &g, int blk, int p, int lt, int k)
+{
+    return (blk * 2 + p) * g.bh + lt * g.run + k;
+}
+
+// storage idx of the r-th site of time slice t (r < Nx Ny Nz): the z-blocks
+// of block row t / bt in order, each contributing its even, then its odd run
+// of bz Nx Ny / 2 sites at lt = t % bt.
 __host__ __device__ inline int slice_index(const Geom &g, int t, int r)
 {
-    const int l3 = g.lb[0] + g.lb[1] + g.lb[2];
-    const int tile = (r >> l3) + (g.tn[0] * g.tn[1] * g.tn[2]) * (t >> g.lb[3]);
-    const int rin = r & ((1 << l3) - 1);
-    const int p = rin >> (l3 - 1);
-    const int jj = rin & ((1 << (l3 - 1)) - 1);
-    const int lt = t & ((1 << g.lb[3]) - 1);
-    return (tile << g.ltv) + (p << (g.ltv - 1)) + (lt << (l3 - 1)) + jj;
+    const int q = fdiv(r, g.d_run);
+    const int j = r - q * g.run;
+    const int blk = (q >> 1) + g.nzb * (t >> g.lbt);
+    return (blk * 2 + (q & 1)) * g.bh + (t & ((1 << g.lbt) - 1)) * g.run + j;
 }
 
 // coordinates of r + s * e_mu (s = +-1), periodic.  Written with selects over

@@ -66,6 +66,36 @@ static void check(const int dims[4])
             hit[idx] = 1;
             REQUIRE(decode(g, idx).c[3] == t, "time of slice_index(%d, %d)", t, r);
         }
+    // the measurement's chunking: block b -> run (blk, lt), chunk jc; lanes
+    // (p, k) cover every site once, all on slice t; partial slots unique
+    {
+        std::vector<char> site(V, 0), slot((size_t)g.nzb * g.cpr * dims[3], 0);
+        const int nblocks = dims[3] * g.nzb * g.cpr;
+        for (int b = 0; b < nblocks; b++) {
+            const int rest = fdiv(b, g.d_cpr), jc = b - rest * g.cpr;
+            const int lt = rest & ((1 << g.lbt) - 1), blk = rest >> g.lbt;
+            const int bt = fdiv(blk, g.d_nzb), zb = blk - bt * g.nzb;
+            const int t = (bt << g.lbt) + lt, c = zb * g.cpr + jc;
+            REQUIRE(t < dims[3] && c < g.nzb * g.cpr, "chunk %d -> t %d c %d", b, t, c);
+            REQUIRE(!slot[(size_t)c * dims[3] + t], "partial slot of chunk %d", b);
+            slot[(size_t)c * dims[3] + t] = 1;
+            for (int lane = 0; lane < 64; lane++) {
+                const int k = jc * 32 + (lane & 31);
+                if (k >= g.run) continue;
+                const int idx = run_index(g, blk, lane >> 5, lt, k);
+                REQUIRE(idx >= 0 && idx < V && !site[idx], "run_index chunk %d lane %d = %d", b, lane, idx);
+                site[idx] = 1;
+                REQUIRE(decode(g, idx).c[3] == t, "time of chunk %d lane %d", b, lane);
+            }
+        }
+        for (int i = 0; i < V; i++) REQUIRE(site[i], "site %d not covered by the measurement chunks", i);
+    }
+    // the flow stage's pairing of the h-th sites of both parities
+    for (int h = 0; h < V / 2; h++)
+        for (int p = 0; p < 2; p++) {
+            const Coord c = decode(g, parity_index(g, p, h));
+            REQUIRE(((c.c[0] + c.c[1] + c.c[2] + c.c[3]) & 1) == p, "flow pairing parity");
+        }
     std::printf("ok %d %d %d %d\n", dims[0], dims[1], dims[2], dims[3]);
 }
 
