@@ -33,24 +33,25 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "indep. configs/s (tau_int Q^2) + flow steps/s at 32^4, FP32/learned vs FP64 plain"
 
 
-def link_bytes(fp32):
-    return 72 if fp32 else 144      # 18 reals per link
+def link_bytes(fp32, rows=3):
+    """Bytes of one stored link: 6 reals per row (3 complex)."""
+    return 6 * rows * (4 if fp32 else 8)
 
 
 def alg_bytes(fp32):
     return 32 if fp32 else 64       # 8 reals per su(3) accumulator entry
 
 
-def flow_stage_bytes_per_launch(V, fp32):
+def flow_stage_bytes_per_launch(V, fp32, rows=3):
     """Per RK3 step the three stage launches read 4V links and write 4V links
     each, and move the su(3) accumulator 4V entries (write), 8V (read+write),
     4V (read): 24 s_L + 16 s_Z per site per step, a third per launch."""
-    return (24 * link_bytes(fp32) + 16 * alg_bytes(fp32)) * V / 3.0
+    return (24 * link_bytes(fp32, rows) + 16 * alg_bytes(fp32)) * V / 3.0
 
 
-def metropolis_bytes_per_launch(V, fp32):
+def metropolis_bytes_per_launch(V, fp32, rows=3):
     """A sub-sweep (one mu, one parity) reads all 4V links and writes V/2."""
-    return 4.5 * V * link_bytes(fp32)
+    return 4.5 * V * link_bytes(fp32, rows)
 
 
 def log(msg):
@@ -66,6 +67,8 @@ def main():
     ap.add_argument("--sweeps", type=int, default=100, help="timed Metropolis sweeps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--precision", choices=["fp32", "fp32c"], default="fp32c",
+                    help="FP32 mode of the timed flow and sweeps: fp32 stores 3 rows per link, fp32c two")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy kernel (for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -95,9 +98,10 @@ def main():
     beta, eps_flow = 6.1, 0.01
 
     # ---------------------------------------------------------------- FP32 flow
-    L = G.Lattice(n, G.FP32_LINKS)
+    rows = 2 if args.precision == "fp32c" else 3
+    L = G.Lattice(n, G.FP32_COMPRESSED if rows == 2 else G.FP32_LINKS)
     L.hot_start(20261202)
-    log(f"[bench] lattice {n}, FP32 links, {L.device_bytes / 2**20:.0f} MiB on device")
+    log(f"[bench] lattice {n}, FP32 links ({rows} stored rows), {L.device_bytes / 2**20:.0f} MiB on device")
     # Metropolis: a few thermalising sweeps, then the timed sweeps
     L.metropolis(5, beta, 0.24, 10, reunit_every=1, seed=7, sweep_offset=0)
     L.synchronize()
@@ -127,14 +131,14 @@ def main():
     L.timing(False)
     L.close()
     steps_per_s = args.steps / elapsed
-    log(f"[bench] flow fp32: {args.steps} steps in {elapsed:.3f} s = {steps_per_s:.1f} steps/s; "
+    log(f"[bench] flow {args.precision}: {args.steps} steps in {elapsed:.3f} s = {steps_per_s:.1f} steps/s; "
         f"stage kernel {f_ms / max(f_launches, 1):.4f} ms x {f_launches}, "
         f"measure {meas_ms / max(meas_launches, 1):.4f} ms x {meas_launches}")
 
-    stage_bytes = flow_stage_bytes_per_launch(V, True)
+    stage_bytes = flow_stage_bytes_per_launch(V, True, rows)
     stage_ms = f_ms / max(f_launches, 1)
     achieved = stage_bytes / (stage_ms * 1e-3) / 1e9
-    metro_bytes = metropolis_bytes_per_launch(V, True)
+    metro_bytes = metropolis_bytes_per_launch(V, True, rows)
     metro_ms = m_ms / max(m_launches, 1)
     metro_gbs = metro_bytes / (metro_ms * 1e-3) / 1e9
 
@@ -187,13 +191,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 links, fp64 exp coefficients and sums",
+        "dtype": ("fp32 links (2 stored rows, third rebuilt)" if rows == 2 else "fp32 links")
+                 + ", fp64 exp coefficients and sums",
         "data": "synthetic: Haar hot start + 105 Metropolis sweeps at beta 6.1, seed 20261202",
         "config": {"workload": "RK3 Wilson flow step + clover measurement", "lattice": list(n),
                    "beta": beta, "flow_eps": eps_flow, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "flow_stage_kernel<float>",
+                     "kernel": f"flow_stage_kernel<float, {rows}>",
                      "alg_bytes_per_launch": stage_bytes, "mean_launch_ms": stage_ms},
         "metropolis": {"sweeps_per_s": sweeps_per_s, "acceptance": float(acc.mean()),
                        "subsweep_kernel_ms": metro_ms, "subsweep_kernel_gbs": metro_gbs,

@@ -18,11 +18,11 @@ import os
 
 import numpy as np
 
-__all__ = ["Lattice", "GlacxError", "library", "FP64", "FP32_LINKS", "MAIN", "FLOW",
+__all__ = ["Lattice", "GlacxError", "library", "FP64", "FP32_LINKS", "FP32_COMPRESSED", "MAIN", "FLOW",
            "K_FLOW_STAGE", "K_MEASURE", "K_METROPOLIS", "K_PLAQUETTE", "K_METROPOLIS_LEARNED", "K_TRAIN",
            "PROPOSAL_NPARAM", "LIB_PATH"]
 
-FP64, FP32_LINKS = 0, 1
+FP64, FP32_LINKS, FP32_COMPRESSED = 0, 1, 2
 MAIN, FLOW = 0, 1
 K_FLOW_STAGE, K_MEASURE, K_METROPOLIS, K_PLAQUETTE, K_METROPOLIS_LEARNED, K_TRAIN = 0, 1, 2, 3, 4, 5
 PROPOSAL_NPARAM = 2468
@@ -60,6 +60,9 @@ def library():
         "glacx_copy_field": ([_vp, C.c_int, C.c_int], C.c_int),
         "glacx_load_bin": ([_vp, C.c_int, C.c_char_p], C.c_int),
         "glacx_save_bin": ([_vp, C.c_int, C.c_char_p], C.c_int),
+        "glacx_load_chroma": ([_vp, C.c_int, C.c_char_p], C.c_int),
+        "glacx_measure_super": ([_vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int),
+        "glacx_flow_super": ([_vp, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int),
         "glacx_cold_start": ([_vp, C.c_int], C.c_int),
         "glacx_hot_start": ([_vp, C.c_int, C.c_uint64], C.c_int),
         "glacx_metropolis": ([_vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_uint64,
@@ -169,6 +172,10 @@ class Lattice:
     def load_bin(self, path, field=MAIN):
         self._check(self._L.glacx_load_bin(self._h, field, os.fsencode(path)), "load_bin")
 
+    def load_chroma(self, path, field=MAIN):
+        """Chroma configuration: GLAC's .bin order with big-endian doubles."""
+        self._check(self._L.glacx_load_chroma(self._h, field, os.fsencode(path)), "load_chroma")
+
     def save_bin(self, path, field=MAIN):
         self._check(self._L.glacx_save_bin(self._h, field, os.fsencode(path)), "save_bin")
 
@@ -194,6 +201,14 @@ class Lattice:
                     "measure")
         return p.value, e.value, q.value, qt
 
+    def measure_super(self, field=MAIN):
+        """GLAC's SuperSampler: (plaq, E_raw, Q, W, Q(t), W(t))."""
+        p, e, q, w = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        qt, wt = np.empty(self.dims[3]), np.empty(self.dims[3])
+        self._check(self._L.glacx_measure_super(self._h, field, C.byref(p), C.byref(e), C.byref(q), C.byref(w),
+                                                _ptr(qt), _ptr(wt)), "measure_super")
+        return p.value, e.value, q.value, w.value, qt, wt
+
     def plaquette(self, field=MAIN) -> float:
         p = C.c_double()
         self._check(self._L.glacx_plaquette(self._h, field, C.byref(p)), "plaquette")
@@ -210,6 +225,16 @@ class Lattice:
         self._check(self._L.glacx_flow(self._h, field, n_steps, eps, _ptr(p), _ptr(e), _ptr(q), _ptr(qt)),
                     "flow")
         return p, e, q, qt
+
+    def flow_super(self, n_steps, eps=0.01, field=FLOW):
+        """RK3 flow in place measuring the SuperSampler observables before
+        the first and after every step: (plaq, E_raw, Q, W, Q(t), W(t))."""
+        m = n_steps + 1
+        p, e, q, w = np.empty(m), np.empty(m), np.empty(m), np.empty(m)
+        qt, wt = np.empty((m, self.dims[3])), np.empty((m, self.dims[3]))
+        self._check(self._L.glacx_flow_super(self._h, field, n_steps, eps, _ptr(p), _ptr(e), _ptr(q), _ptr(w),
+                                             _ptr(qt), _ptr(wt)), "flow_super")
+        return p, e, q, w, qt, wt
 
     def action_derivative(self, mu, field=MAIN) -> np.ndarray:
         Z = np.empty((self.V, 18))

@@ -3,6 +3,7 @@
 // drivers that sequence the kernels on the context's stream.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
@@ -147,28 +148,28 @@ int timed_end(glacx_ctx *c, int kernel, hipEvent_t start)
 }
 
 // ---- kernel sequences ----------------------------------------------------
-template <class T> int run_measure(glacx_ctx *c, const void *f, double *out)
+template <class T, int R> int run_measure(glacx_ctx *c, const void *f, double *out, bool super)
 {
     int rc = ensure(c, &c->partials, &c->partials_len, (size_t)measure_partials_count(c->geom));
     if (rc) return rc;
     hipEvent_t e0 = nullptr;
     if ((rc = timed_begin(c, &e0))) return rc;
-    launch_measure<T>(f, c->geom, c->partials, c->stream);
+    launch_measure<T, R>(f, c->geom, c->partials, super, c->stream);
     CHECK_LAUNCH(c);
     if ((rc = timed_end(c, GLACX_K_MEASURE, e0))) return rc;
-    launch_measure_finish(c->partials, c->geom, out, c->stream);
+    launch_measure_finish(c->partials, c->geom, out, super, c->stream);
     CHECK_LAUNCH(c);
     return GLACX_OK;
 }
 
-template <class T> int run_plaquette(glacx_ctx *c, const void *f, double *out)
+template <class T, int R> int run_plaquette(glacx_ctx *c, const void *f, double *out)
 {
     const int np = plaquette_partials_count(c->geom);
     int rc = ensure(c, &c->partials, &c->partials_len, (size_t)np);
     if (rc) return rc;
     hipEvent_t e0 = nullptr;
     if ((rc = timed_begin(c, &e0))) return rc;
-    launch_plaquette<T>(f, c->geom, c->partials, c->stream);
+    launch_plaquette<T, R>(f, c->geom, c->partials, c->stream);
     CHECK_LAUNCH(c);
     if ((rc = timed_end(c, GLACX_K_PLAQUETTE, e0))) return rc;
     launch_plaquette_finish(c->partials, np, c->geom, out, c->stream);
@@ -178,28 +179,28 @@ template <class T> int run_plaquette(glacx_ctx *c, const void *f, double *out)
 
 // One RK3 step on `f` (result back in `f`): three stages ping-ponging
 // through the work buffer, then a device copy back for the odd stage count.
-template <class T> int run_flow_step(glacx_ctx *c, void *f, double eps)
+template <class T, int R> int run_flow_step(glacx_ctx *c, void *f, double eps)
 {
     void *bufs[2] = {f, c->work};
     int rc;
     for (int s = 0; s < 3; s++) {
         hipEvent_t e0 = nullptr;
         if ((rc = timed_begin(c, &e0))) return rc;
-        launch_flow_stage<T>(bufs[s & 1], bufs[(s + 1) & 1], c->acc, c->geom, s, eps, c->stream);
+        launch_flow_stage<T, R>(bufs[s & 1], bufs[(s + 1) & 1], c->acc, c->geom, s, eps, c->stream);
         CHECK_LAUNCH(c);
         if ((rc = timed_end(c, GLACX_K_FLOW_STAGE, e0))) return rc;
     }
     return GLACX_OK;
 }
 
-template <class T> int run_metropolis_sweep(glacx_ctx *c, const MetroParams &p, unsigned long long *count)
+template <class T, int R> int run_metropolis_sweep(glacx_ctx *c, const MetroParams &p, unsigned long long *count)
 {
     int rc;
     for (int mu = 0; mu < 4; mu++)
         for (int parity = 0; parity < 2; parity++) {
             hipEvent_t e0 = nullptr;
             if ((rc = timed_begin(c, &e0))) return rc;
-            launch_metropolis<T>(c->field[GLACX_FIELD_MAIN], c->geom, mu, parity, p, count, c->stream);
+            launch_metropolis<T, R>(c->field[GLACX_FIELD_MAIN], c->geom, mu, parity, p, count, c->stream);
             CHECK_LAUNCH(c);
             if ((rc = timed_end(c, GLACX_K_METROPOLIS, e0))) return rc;
         }
@@ -224,14 +225,14 @@ int ensure_proposal_buffers(glacx_ctx *c)
     return GLACX_OK;
 }
 
-template <class T> int run_metropolis_learned_sweep(glacx_ctx *c, const MetroParams &p, unsigned long long *count)
+template <class T, int R> int run_metropolis_learned_sweep(glacx_ctx *c, const MetroParams &p, unsigned long long *count)
 {
     int rc;
     for (int mu = 0; mu < 4; mu++)
         for (int parity = 0; parity < 2; parity++) {
             hipEvent_t e0 = nullptr;
             if ((rc = timed_begin(c, &e0))) return rc;
-            launch_metropolis_learned<T>(c->field[GLACX_FIELD_MAIN], c->geom, mu, parity, p, c->params_bf16,
+            launch_metropolis_learned<T, R>(c->field[GLACX_FIELD_MAIN], c->geom, mu, parity, p, c->params_bf16,
                                          c->mean_std, count, c->stream);
             CHECK_LAUNCH(c);
             if ((rc = timed_end(c, GLACX_K_METROPOLIS_LEARNED, e0))) return rc;
@@ -239,13 +240,13 @@ template <class T> int run_metropolis_learned_sweep(glacx_ctx *c, const MetroPar
     return GLACX_OK;
 }
 
-template <class T> int run_collect(glacx_ctx *c, int stride)
+template <class T, int R> int run_collect(glacx_ctx *c, int stride)
 {
     const size_t per = (size_t)((c->geom.vh + stride - 1) / stride);
     for (int mu = 0; mu < 4; mu++)
         for (int parity = 0; parity < 2; parity++) {
             if (c->samples_n >= c->samples_cap) return GLACX_OK;
-            launch_collect<T>(c->field[GLACX_FIELD_MAIN], c->geom, mu, parity, stride, c->samples, c->samples_cap,
+            launch_collect<T, R>(c->field[GLACX_FIELD_MAIN], c->geom, mu, parity, stride, c->samples, c->samples_cap,
                               c->samples_n, c->stream);
             CHECK_LAUNCH(c);
             c->samples_n = std::min(c->samples_cap, c->samples_n + per);
@@ -261,7 +262,55 @@ template <class T> int run_collect(glacx_ctx *c, int stride)
         HIP_TRY(c, hipSetDevice((c)->device));                                                           \
     } while (0)
 
-#define DISPATCH(c, fn, ...) ((c)->precision == GLACX_FP32_LINKS ? fn<float>(__VA_ARGS__) : fn<double>(__VA_ARGS__))
+// Kernel instantiation for the context's precision: <real type, stored rows>.
+#define DISPATCH(c, fn, ...)                                                                             \
+    ((c)->precision == GLACX_FP32_LINKS        ? fn<float, 3>(__VA_ARGS__)                                \
+     : (c)->precision == GLACX_FP32_COMPRESSED ? fn<float, 2>(__VA_ARGS__)                                \
+                                               : fn<double, 3>(__VA_ARGS__))
+
+namespace {
+
+// The compressed mode stores two rows and rebuilds the third, which is only
+// right for SU(3) links; refuse anything else (e.g. det = -1 links) at the
+// door.  `link` points at 18 doubles in GLAC's layout.
+double su3_defect(const double *m)
+{
+    double worst = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double re = 0.0, im = 0.0;
+            for (int k = 0; k < 3; k++) {
+                re += m[6 * i + 2 * k] * m[6 * j + 2 * k] + m[6 * i + 2 * k + 1] * m[6 * j + 2 * k + 1];
+                im += m[6 * i + 2 * k + 1] * m[6 * j + 2 * k] - m[6 * i + 2 * k] * m[6 * j + 2 * k + 1];
+            }
+            worst = std::max(worst, std::fabs(re - (i == j ? 1.0 : 0.0)) + std::fabs(im));
+        }
+    // third row against conj(row0 x row1)
+    for (int j = 0; j < 3; j++) {
+        const int k = (j + 1) % 3, l = (j + 2) % 3;
+        const double ar = m[2 * k], ai = m[2 * k + 1], br = m[6 + 2 * l], bi = m[6 + 2 * l + 1];
+        const double cr = m[2 * l], ci = m[2 * l + 1], dr = m[6 + 2 * k], di = m[6 + 2 * k + 1];
+        const double re = ar * br - ai * bi - (cr * dr - ci * di);
+        const double im = ar * bi + ai * br - (cr * di + ci * dr);
+        worst = std::max(worst, std::fabs(m[12 + 2 * j] - re) + std::fabs(m[12 + 2 * j + 1] + im));
+    }
+    return worst;
+}
+
+int require_su3(glacx_ctx *c, const double *links, size_t n_links, size_t stride, const char *who)
+{
+    if (c->precision != GLACX_FP32_COMPRESSED) return GLACX_OK;
+    for (size_t i = 0; i < n_links; i++) {
+        const double d = su3_defect(links + i * stride);
+        if (!(d < 1e-6))
+            return fail(c, GLACX_E_ARG, std::string(who) + ": the compressed FP32 mode stores two rows per link and "
+                                        "needs SU(3) links; link " + std::to_string(i) + " deviates by " +
+                                        std::to_string(d));
+    }
+    return GLACX_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -275,14 +324,15 @@ int glacx_create(const int dims[4], int precision, int device, glacx_ctx **out)
         v *= dims[k];
     }
     // the 9 planes of one direction are addressed by one buffer resource
-    if ((unsigned long long)v * 9ull * 2ull * (precision == GLACX_FP32_LINKS ? 4ull : 8ull) >= (1ull << 32))
+    if (precision != GLACX_FP64 && precision != GLACX_FP32_LINKS && precision != GLACX_FP32_COMPRESSED)
         return GLACX_E_ARG;
-    if (precision != GLACX_FP64 && precision != GLACX_FP32_LINKS) return GLACX_E_ARG;
+    if ((unsigned long long)v * 9ull * 2ull * (precision != GLACX_FP64 ? 4ull : 8ull) >= (1ull << 32))
+        return GLACX_E_ARG;
     glacx_ctx *c = new glacx_ctx;
     for (int k = 0; k < 4; k++) c->dims[k] = dims[k];
     c->geom = make_geom(dims);
     c->precision = precision;
-    c->real_size = precision == GLACX_FP32_LINKS ? 4 : 8;
+    c->real_size = precision != GLACX_FP64 ? 4 : 8;
     c->device = device;
     int rc = GLACX_OK;
     auto bad = [&](int code) {
@@ -338,8 +388,10 @@ int glacx_upload_field(glacx_ctx *c, int field, const double *const links[4])
     if (!valid_field(field) || !links) return fail(c, GLACX_E_ARG, "upload: bad field or pointer");
     for (int mu = 0; mu < 4; mu++)
         if (!links[mu]) return fail(c, GLACX_E_ARG, "upload: null link array");
-    int rc = ensure_stage(c);
-    if (rc) return rc;
+    int rc;
+    for (int mu = 0; mu < 4; mu++)
+        if ((rc = require_su3(c, links[mu], (size_t)c->geom.v, 18, "upload"))) return rc;
+    if ((rc = ensure_stage(c))) return rc;
     const size_t n = (size_t)c->geom.v * 18;
     for (int mu = 0; mu < 4; mu++)
         HIP_TRY(c, hipMemcpyAsync(c->stage + mu * n, links[mu], n * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -375,24 +427,49 @@ int glacx_copy_field(glacx_ctx *c, int dst, int src)
     return GLACX_OK;
 }
 
-int glacx_load_bin(glacx_ctx *c, int field, const char *path)
+namespace {
+
+// Reads V * 72 doubles of GLAC's .bin site order and uploads them; with
+// `byteswap` every double is converted from big-endian first (Chroma files).
+int load_site_major(glacx_ctx *c, int field, const char *path, bool byteswap, const char *who)
 {
-    CTX_CHECK(c);
-    if (!valid_field(field) || !path) return fail(c, GLACX_E_ARG, "load_bin: bad field or path");
+    if (!valid_field(field) || !path) return fail(c, GLACX_E_ARG, std::string(who) + ": bad field or path");
     const size_t n = (size_t)c->geom.v * 72;
     FILE *fp = std::fopen(path, "rb");
-    if (!fp) return fail(c, GLACX_E_IO, std::string("load_bin: cannot open ") + path);
+    if (!fp) return fail(c, GLACX_E_IO, std::string(who) + ": cannot open " + path);
     std::vector<double> buf(n);
     const size_t got = std::fread(buf.data(), sizeof(double), n, fp);
     std::fclose(fp);
-    if (got != n) return fail(c, GLACX_E_IO, std::string("load_bin: short file ") + path);
-    int rc = ensure_stage(c);
+    if (got != n) return fail(c, GLACX_E_IO, std::string(who) + ": short file " + path);
+    if (byteswap)
+        for (size_t i = 0; i < n; i++) {
+            uint64_t u;
+            std::memcpy(&u, &buf[i], sizeof u);
+            u = __builtin_bswap64(u);
+            std::memcpy(&buf[i], &u, sizeof u);
+        }
+    int rc = require_su3(c, buf.data(), (size_t)c->geom.v * 4, 18, who);
     if (rc) return rc;
+    if ((rc = ensure_stage(c))) return rc;
     HIP_TRY(c, hipMemcpyAsync(c->stage, buf.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     DISPATCH(c, launch_pack, c->stage, c->field[field], c->geom, 1, c->stream);
     CHECK_LAUNCH(c);
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return GLACX_OK;
+}
+
+}  // namespace
+
+int glacx_load_bin(glacx_ctx *c, int field, const char *path)
+{
+    CTX_CHECK(c);
+    return load_site_major(c, field, path, false, "load_bin");
+}
+
+int glacx_load_chroma(glacx_ctx *c, int field, const char *path)
+{
+    CTX_CHECK(c);
+    return load_site_major(c, field, path, true, "load_chroma");
 }
 
 int glacx_save_bin(glacx_ctx *c, int field, const char *path)
@@ -467,20 +544,44 @@ int glacx_metropolis(glacx_ctx *c, int n_sweeps, double beta, double eps, int n_
     return GLACX_OK;
 }
 
-int glacx_measure(glacx_ctx *c, int field, double *plaq, double *e_raw, double *q, double *q_t)
+namespace {
+
+// Clover observables of `field`; with `super` also GLAC's Weinberg operator.
+int measure_impl(glacx_ctx *c, int field, bool super, double *plaq, double *e_raw, double *q, double *w, double *q_t,
+                 double *w_t)
 {
-    CTX_CHECK(c);
     if (!valid_field(field)) return fail(c, GLACX_E_ARG, "measure: bad field");
-    int rc = DISPATCH(c, run_measure, c, c->field[field], c->results);
+    const int nt = c->dims[3];
+    const size_t row = super ? (size_t)4 + 2 * nt : (size_t)3 + nt;
+    int rc = ensure(c, &c->results, &c->results_len, row);
     if (rc) return rc;
-    std::vector<double> h((size_t)3 + c->dims[3]);
+    if ((rc = DISPATCH(c, run_measure, c, c->field[field], c->results, super))) return rc;
+    std::vector<double> h(row);
     HIP_TRY(c, hipMemcpyAsync(h.data(), c->results, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const int head = super ? 4 : 3;
     if (plaq) *plaq = h[0];
     if (e_raw) *e_raw = h[1];
     if (q) *q = h[2];
-    if (q_t) std::memcpy(q_t, h.data() + 3, sizeof(double) * c->dims[3]);
+    if (w) *w = h[3];
+    if (q_t) std::memcpy(q_t, h.data() + head, sizeof(double) * nt);
+    if (w_t) std::memcpy(w_t, h.data() + head + nt, sizeof(double) * nt);
     return GLACX_OK;
+}
+
+}  // namespace
+
+int glacx_measure(glacx_ctx *c, int field, double *plaq, double *e_raw, double *q, double *q_t)
+{
+    CTX_CHECK(c);
+    return measure_impl(c, field, false, plaq, e_raw, q, nullptr, q_t, nullptr);
+}
+
+int glacx_measure_super(glacx_ctx *c, int field, double *plaq, double *e_raw, double *q, double *w, double *q_t,
+                        double *w_t)
+{
+    CTX_CHECK(c);
+    return measure_impl(c, field, true, plaq, e_raw, q, w, q_t, w_t);
 }
 
 int glacx_plaquette(glacx_ctx *c, int field, double *plaq)
@@ -494,38 +595,59 @@ int glacx_plaquette(glacx_ctx *c, int field, double *plaq)
     return GLACX_OK;
 }
 
-int glacx_flow(glacx_ctx *c, int field, int n_steps, double eps, double *plaq, double *e_raw, double *q, double *q_t)
+namespace {
+
+int flow_impl(glacx_ctx *c, int field, int n_steps, double eps, bool super, double *plaq, double *e_raw, double *q,
+              double *w, double *q_t, double *w_t)
 {
-    CTX_CHECK(c);
     if (!valid_field(field) || n_steps < 0) return fail(c, GLACX_E_ARG, "flow: bad field or step count");
-    const bool meas = plaq || e_raw || q || q_t;
-    const size_t row = (size_t)3 + c->dims[3];
+    const bool meas = plaq || e_raw || q || q_t || w || w_t;
+    const int nt = c->dims[3];
+    const size_t row = super ? (size_t)4 + 2 * nt : (size_t)3 + nt;
     int rc;
     if (meas && (rc = ensure(c, &c->results, &c->results_len, row * (size_t)(n_steps + 1)))) return rc;
     void *f = c->field[field];
-    if (meas && (rc = DISPATCH(c, run_measure, c, f, c->results))) return rc;
+    if (meas && (rc = DISPATCH(c, run_measure, c, f, c->results, super))) return rc;
     for (int i = 0; i < n_steps; i++) {
         if ((rc = DISPATCH(c, run_flow_step, c, f, eps))) return rc;
         // three stages leave the result in the work buffer: swap roles
         std::swap(c->field[field], c->work);
         f = c->field[field];
-        if (meas && (rc = DISPATCH(c, run_measure, c, f, c->results + row * (size_t)(i + 1)))) return rc;
+        if (meas && (rc = DISPATCH(c, run_measure, c, f, c->results + row * (size_t)(i + 1), super))) return rc;
     }
     if (meas) {
         std::vector<double> h(row * (size_t)(n_steps + 1));
         HIP_TRY(c, hipMemcpyAsync(h.data(), c->results, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
+        const int head = super ? 4 : 3;
         for (int i = 0; i <= n_steps; i++) {
             const double *r = h.data() + row * i;
             if (plaq) plaq[i] = r[0];
             if (e_raw) e_raw[i] = r[1];
             if (q) q[i] = r[2];
-            if (q_t) std::memcpy(q_t + (size_t)i * c->dims[3], r + 3, sizeof(double) * c->dims[3]);
+            if (w) w[i] = r[3];
+            if (q_t) std::memcpy(q_t + (size_t)i * nt, r + head, sizeof(double) * nt);
+            if (w_t) std::memcpy(w_t + (size_t)i * nt, r + head + nt, sizeof(double) * nt);
         }
     } else {
         HIP_TRY(c, hipStreamSynchronize(c->stream));
     }
     return GLACX_OK;
+}
+
+}  // namespace
+
+int glacx_flow(glacx_ctx *c, int field, int n_steps, double eps, double *plaq, double *e_raw, double *q, double *q_t)
+{
+    CTX_CHECK(c);
+    return flow_impl(c, field, n_steps, eps, false, plaq, e_raw, q, nullptr, q_t, nullptr);
+}
+
+int glacx_flow_super(glacx_ctx *c, int field, int n_steps, double eps, double *plaq, double *e_raw, double *q,
+                     double *w, double *q_t, double *w_t)
+{
+    CTX_CHECK(c);
+    return flow_impl(c, field, n_steps, eps, true, plaq, e_raw, q, w, q_t, w_t);
 }
 
 int glacx_action_derivative(glacx_ctx *c, int field, int mu, double *z_out)

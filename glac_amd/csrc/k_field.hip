@@ -12,7 +12,7 @@
 
 namespace glacx {
 
-template <class T>
+template <class T, int R>
 __global__ __launch_bounds__(kBlock) void pack_kernel(const double *__restrict__ stage,
                                                       typename vec2<T>::type *__restrict__ f, Geom g, int site_major)
 {
@@ -25,11 +25,11 @@ __global__ __launch_bounds__(kBlock) void pack_kernel(const double *__restrict__
         M3<T> m;
 #pragma unroll
         for (int k = 0; k < 9; k++) m.e[k / 3][k % 3] = {T(src[2 * k]), T(src[2 * k + 1])};
-        store_link<T>(f, g.v, mu, idx, m);
+        store_link<T, R>(f, g, mu, idx, m);
     }
 }
 
-template <class T>
+template <class T, int R>
 __global__ __launch_bounds__(kBlock) void unpack_kernel(const typename vec2<T>::type *__restrict__ f,
                                                         double *__restrict__ stage, Geom g, int site_major)
 {
@@ -39,7 +39,7 @@ __global__ __launch_bounds__(kBlock) void unpack_kernel(const typename vec2<T>::
 #pragma unroll
     for (int mu = 0; mu < 4; mu++) {
         double *dst = site_major ? stage + ((size_t)lex * 4 + mu) * 18 : stage + ((size_t)mu * g.v + lex) * 18;
-        const M3<T> m = load_link<T>(f, g.v, mu, idx);
+        const M3<T> m = load_link<T, R>(f, g, mu, idx);
 #pragma unroll
         for (int k = 0; k < 9; k++) {
             dst[2 * k] = double(m.e[k / 3][k % 3].r);
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kBlock) void unpack_kernel(const typename vec2<T>::
 
 // One thread per link: 9 complex Gaussians by Box-Muller (entry k from stream
 // uniforms 2k, 2k+1), then FP64 row Gram-Schmidt with row 2 = conj(r0 x r1).
-template <class T>
+template <class T, int R>
 __global__ __launch_bounds__(kBlock) void hot_start_kernel(typename vec2<T>::type *__restrict__ f, Geom g, uint64_t seed)
 {
     const int t = blockIdx.x * kBlock + threadIdx.x;
@@ -68,27 +68,27 @@ __global__ __launch_bounds__(kBlock) void hot_start_kernel(typename vec2<T>::typ
         m.e[k / 3][k % 3] = {r * c, r * s};
     }
     reunitarize(m);
-    store_link_any<T>(f, g.v, mu, idx, convert<T>(m));
+    store_link_any<T, R>(f, g, mu, idx, convert<T>(m));
 }
 
-template <class T>
+template <class T, int R>
 __global__ __launch_bounds__(kBlock) void cold_start_kernel(typename vec2<T>::type *__restrict__ f, Geom g)
 {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= 4 * g.v) return;
     const int mu = t / g.v, idx = t - mu * g.v;
-    store_link_any<T>(f, g.v, mu, idx, unit3<T>());
+    store_link_any<T, R>(f, g, mu, idx, unit3<T>());
 }
 
-template <class T>
+template <class T, int R>
 __global__ __launch_bounds__(kBlock) void action_derivative_kernel(const typename vec2<T>::type *__restrict__ f,
                                                                    double *__restrict__ out, Geom g, int mu)
 {
     const int idx = blockIdx.x * kBlock + threadIdx.x;
     if (idx >= g.v) return;
     const Coord x = decode(g, idx);
-    const M3<T> a = staple<T>(f, g, x, mu);
-    const M3<T> u = load_link<T>(f, g.v, mu, idx);
+    const M3<T> a = staple<T, R>(f, g, x, mu);
+    const M3<T> u = load_link<T, R>(f, g, mu, idx);
     const M3<T> z = expand(force_from_omega(mul(u, a)));
     double *dst = out + (size_t)lex_of(g, x) * 18;
 #pragma unroll
@@ -100,34 +100,35 @@ __global__ __launch_bounds__(kBlock) void action_derivative_kernel(const typenam
 
 static inline unsigned blocks_for(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-template <class T> void launch_pack(const double *stage, void *f, const Geom &g, int site_major, hipStream_t s)
+template <class T, int R> void launch_pack(const double *stage, void *f, const Geom &g, int site_major, hipStream_t s)
 {
-    pack_kernel<T><<<blocks_for(g.v), kBlock, 0, s>>>(stage, (typename vec2<T>::type *)f, g, site_major);
+    pack_kernel<T, R><<<blocks_for(g.v), kBlock, 0, s>>>(stage, (typename vec2<T>::type *)f, g, site_major);
 }
-template <class T> void launch_unpack(const void *f, double *stage, const Geom &g, int site_major, hipStream_t s)
+template <class T, int R> void launch_unpack(const void *f, double *stage, const Geom &g, int site_major, hipStream_t s)
 {
-    unpack_kernel<T><<<blocks_for(g.v), kBlock, 0, s>>>((const typename vec2<T>::type *)f, stage, g, site_major);
+    unpack_kernel<T, R><<<blocks_for(g.v), kBlock, 0, s>>>((const typename vec2<T>::type *)f, stage, g, site_major);
 }
-template <class T> void launch_hot_start(void *f, const Geom &g, uint64_t seed, hipStream_t s)
+template <class T, int R> void launch_hot_start(void *f, const Geom &g, uint64_t seed, hipStream_t s)
 {
-    hot_start_kernel<T><<<blocks_for(4L * g.v), kBlock, 0, s>>>((typename vec2<T>::type *)f, g, seed);
+    hot_start_kernel<T, R><<<blocks_for(4L * g.v), kBlock, 0, s>>>((typename vec2<T>::type *)f, g, seed);
 }
-template <class T> void launch_cold_start(void *f, const Geom &g, hipStream_t s)
+template <class T, int R> void launch_cold_start(void *f, const Geom &g, hipStream_t s)
 {
-    cold_start_kernel<T><<<blocks_for(4L * g.v), kBlock, 0, s>>>((typename vec2<T>::type *)f, g);
+    cold_start_kernel<T, R><<<blocks_for(4L * g.v), kBlock, 0, s>>>((typename vec2<T>::type *)f, g);
 }
-template <class T> void launch_action_derivative(const void *f, double *stage, const Geom &g, int mu, hipStream_t s)
+template <class T, int R> void launch_action_derivative(const void *f, double *stage, const Geom &g, int mu, hipStream_t s)
 {
-    action_derivative_kernel<T><<<blocks_for(g.v), kBlock, 0, s>>>((const typename vec2<T>::type *)f, stage, g, mu);
+    action_derivative_kernel<T, R><<<blocks_for(g.v), kBlock, 0, s>>>((const typename vec2<T>::type *)f, stage, g, mu);
 }
 
-#define INSTANTIATE(T)                                                                                   \
-    template void launch_pack<T>(const double *, void *, const Geom &, int, hipStream_t);                \
-    template void launch_unpack<T>(const void *, double *, const Geom &, int, hipStream_t);              \
-    template void launch_hot_start<T>(void *, const Geom &, uint64_t, hipStream_t);                      \
-    template void launch_cold_start<T>(void *, const Geom &, hipStream_t);                               \
-    template void launch_action_derivative<T>(const void *, double *, const Geom &, int, hipStream_t);
-INSTANTIATE(float)
-INSTANTIATE(double)
+#define INSTANTIATE(T, R)                                                                                \
+    template void launch_pack<T, R>(const double *, void *, const Geom &, int, hipStream_t);             \
+    template void launch_unpack<T, R>(const void *, double *, const Geom &, int, hipStream_t);           \
+    template void launch_hot_start<T, R>(void *, const Geom &, uint64_t, hipStream_t);                   \
+    template void launch_cold_start<T, R>(void *, const Geom &, hipStream_t);                            \
+    template void launch_action_derivative<T, R>(const void *, double *, const Geom &, int, hipStream_t);
+INSTANTIATE(float, 3)
+INSTANTIATE(float, 2)
+INSTANTIATE(double, 3)
 
 } // namespace glacx

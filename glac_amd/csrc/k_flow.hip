@@ -27,7 +27,7 @@
 
 namespace glacx {
 
-template <class T, int STAGE>
+template <class T, int R, int STAGE>
 __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<T>::type *__restrict__ win,
                                                             typename vec2<T>::type *__restrict__ wout,
                                                             T *__restrict__ acc, Geom g, T cz, T ca)
@@ -38,8 +38,8 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
     if (h >= g.vh) return;
     const int idx = parity_index(g, lane >> 5, h);
     const Coord x = decode(g, idx);
-    const M3<T> a = staple<T>(win, g, x, mu);
-    const M3<T> u = load_link<T>(win, g.v, mu, idx);
+    const M3<T> a = staple<T, R>(win, g, x, mu);
+    const M3<T> u = load_link<T, R>(win, g, mu, idx);
     const su3alg<T> z = force_from_omega(mul(u, a));
     T *ap = acc + (size_t)mu * 8 * g.v + idx;
     su3alg<T> y;
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
             for (int k = 0; k < 6; k++) ap[(size_t)(k + 2) * g.v] = y.p[k];
         }
     }
-    store_link<T>(wout, g.v, mu, idx, mul(exp_su3(y), u));
+    store_link<T, R>(wout, g, mu, idx, mul(exp_su3(y), u));
 }
 
 // Dynamic LDS reserved per block; nonzero values cap the blocks resident per
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
 #define GLACX_FLOW_LDS_BYTES 0
 #endif
 
-template <class T>
+template <class T, int R>
 void launch_flow_stage(const void *win, void *wout, void *acc, const Geom &g, int stage, double eps, hipStream_t s)
 {
     using V2 = typename vec2<T>::type;
@@ -83,18 +83,19 @@ void launch_flow_stage(const void *win, void *wout, void *acc, const Geom &g, in
     T *a = (T *)acc;
     switch (stage) {
     case 0:
-        flow_stage_kernel<T, 0><<<blocks, kBlock, GLACX_FLOW_LDS_BYTES, s>>>(in, out, a, g, T(0.25 * eps), T(0));
+        flow_stage_kernel<T, R, 0><<<blocks, kBlock, GLACX_FLOW_LDS_BYTES, s>>>(in, out, a, g, T(0.25 * eps), T(0));
         break;
     case 1:
-        flow_stage_kernel<T, 1><<<blocks, kBlock, GLACX_FLOW_LDS_BYTES, s>>>(in, out, a, g, T(8.0 / 9.0 * eps), T(17.0 / 36.0 * eps));
+        flow_stage_kernel<T, R, 1><<<blocks, kBlock, GLACX_FLOW_LDS_BYTES, s>>>(in, out, a, g, T(8.0 / 9.0 * eps), T(17.0 / 36.0 * eps));
         break;
     default:
-        flow_stage_kernel<T, 2><<<blocks, kBlock, GLACX_FLOW_LDS_BYTES, s>>>(in, out, a, g, T(0.75 * eps), T(1));
+        flow_stage_kernel<T, R, 2><<<blocks, kBlock, GLACX_FLOW_LDS_BYTES, s>>>(in, out, a, g, T(0.75 * eps), T(1));
         break;
     }
 }
 
-template void launch_flow_stage<float>(const void *, void *, void *, const Geom &, int, double, hipStream_t);
-template void launch_flow_stage<double>(const void *, void *, void *, const Geom &, int, double, hipStream_t);
+template void launch_flow_stage<float, 3>(const void *, void *, void *, const Geom &, int, double, hipStream_t);
+template void launch_flow_stage<float, 2>(const void *, void *, void *, const Geom &, int, double, hipStream_t);
+template void launch_flow_stage<double, 3>(const void *, void *, void *, const Geom &, int, double, hipStream_t);
 
 } // namespace glacx

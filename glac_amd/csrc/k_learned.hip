@@ -19,10 +19,11 @@
 
 namespace glacx {
 
-constexpr int kSampleCols = 36;  // A (18 reals) then U (18 reals)
-// per-sample columns kept for the gradient: x, h1, h2, do, da2, da1
+// Samples hold 36 columns per sample: A (18 reals) then U (18 reals).
+// Per-sample columns kept for the gradient: x, h1, h2, do, da2, da1.
 constexpr int kColX = 0, kColH1 = kColX + kMlpIn, kColH2 = kColH1 + kMlpH1, kColDo = kColH2 + kMlpH2,
-              kColDa2 = kColDo + kMlpOut, kColDa1 = kColDa2 + kMlpH2, kInterCols = kColDa1 + kMlpH1;
+              kColDa2 = kColDo + kMlpOut, kColDa1 = kColDa2 + kMlpH2;
+static_assert(kColDa1 + kMlpH1 == kTrainInterCols, "intermediate column count");
 
 template <class T> __device__ __forceinline__ M3<float> to_float(const M3<T> &m) { return convert<float>(m); }
 
@@ -35,7 +36,7 @@ __device__ __forceinline__ M3<float> read_matrix(const float *__restrict__ s, si
 }
 
 // ---- sample collection -----------------------------------------------------
-template <class T>
+template <class T, int R>
 __global__ __launch_bounds__(kBlock) void collect_kernel(const typename vec2<T>::type *__restrict__ f, Geom g, int mu,
                                                          int parity, int stride, float *__restrict__ out, size_t cap,
                                                          size_t base)
@@ -46,8 +47,8 @@ __global__ __launch_bounds__(kBlock) void collect_kernel(const typename vec2<T>:
     if (s >= cap) return;
     const int idx = parity_index(g, parity, h);
     const Coord x = decode(g, idx);
-    const M3<T> a = staple<T>(f, g, x, mu);
-    const M3<T> u = load_link<T>(f, g.v, mu, idx);
+    const M3<T> a = staple<T, R>(f, g, x, mu);
+    const M3<T> u = load_link<T, R>(f, g, mu, idx);
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         out[(size_t)(2 * k) * cap + s] = float(a.e[k / 3][k % 3].r);
@@ -310,7 +311,7 @@ __global__ __launch_bounds__(kBlock) void to_bf16_kernel(const float *__restrict
 }
 
 // ---- sampling with the learned proposal ----------------------------------------
-template <class T>
+template <class T, int R>
 __global__ __launch_bounds__(kBlock) void metropolis_learned_kernel(typename vec2<T>::type *__restrict__ f, Geom g,
                                                                     int mu, int parity, MetroParams prm,
                                                                     const unsigned short *__restrict__ wbf16,
@@ -325,8 +326,8 @@ __global__ __launch_bounds__(kBlock) void metropolis_learned_kernel(typename vec
     if (h < g.vh) {
         const int idx = parity_index(g, parity, h);
         const Coord x = decode(g, idx);
-        const M3<T> a = staple<T>(f, g, x, mu);
-        M3<T> u = load_link<T>(f, g.v, mu, idx);
+        const M3<T> a = staple<T, R>(f, g, x, mu);
+        M3<T> u = load_link<T, R>(f, g, mu, idx);
         float fe[6], o[36], L[36];
         staple_features(a, prm.beta, fe);
         const bool learned = mlp_forward(P, mean_std, mean_std + 6, fe, o);
@@ -369,7 +370,7 @@ __global__ __launch_bounds__(kBlock) void metropolis_learned_kernel(typename vec
             reunitarize(d);
             u = convert<T>(d);
         }
-        store_link<T>(f, g.v, mu, idx, u);
+        store_link<T, R>(f, g, mu, idx, u);
         nacc[0] = (double)accepted;
     }
     block_sum<1>(nacc);
@@ -377,12 +378,12 @@ __global__ __launch_bounds__(kBlock) void metropolis_learned_kernel(typename vec
 }
 
 // ---- launchers -------------------------------------------------------------------
-template <class T>
+template <class T, int R>
 void launch_collect(const void *f, const Geom &g, int mu, int parity, int stride, float *out, size_t cap, size_t base,
                     hipStream_t s)
 {
     const int n = (g.vh + stride - 1) / stride;
-    collect_kernel<T><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>((const typename vec2<T>::type *)f, g, mu, parity,
+    collect_kernel<T, R><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>((const typename vec2<T>::type *)f, g, mu, parity,
                                                                    stride, out, cap, base);
 }
 
@@ -415,22 +416,26 @@ void launch_to_bf16(const float *params, unsigned short *bf16, hipStream_t s)
     to_bf16_kernel<<<(kNParam + kBlock - 1) / kBlock, kBlock, 0, s>>>(params, bf16);
 }
 
-template <class T>
+template <class T, int R>
 void launch_metropolis_learned(void *f, const Geom &g, int mu, int parity, const MetroParams &p,
                                const unsigned short *wbf16, const float *mean_std, unsigned long long *count,
                                hipStream_t s)
 {
     const unsigned blocks = (unsigned)((g.vh + kBlock - 1) / kBlock);
-    metropolis_learned_kernel<T><<<blocks, kBlock, 0, s>>>((typename vec2<T>::type *)f, g, mu, parity, p, wbf16,
+    metropolis_learned_kernel<T, R><<<blocks, kBlock, 0, s>>>((typename vec2<T>::type *)f, g, mu, parity, p, wbf16,
                                                            mean_std, count);
 }
 
-template void launch_collect<float>(const void *, const Geom &, int, int, int, float *, size_t, size_t, hipStream_t);
-template void launch_collect<double>(const void *, const Geom &, int, int, int, float *, size_t, size_t, hipStream_t);
-template void launch_metropolis_learned<float>(void *, const Geom &, int, int, const MetroParams &,
+template void launch_collect<float, 3>(const void *, const Geom &, int, int, int, float *, size_t, size_t, hipStream_t);
+template void launch_collect<float, 2>(const void *, const Geom &, int, int, int, float *, size_t, size_t, hipStream_t);
+template void launch_collect<double, 3>(const void *, const Geom &, int, int, int, float *, size_t, size_t, hipStream_t);
+template void launch_metropolis_learned<float, 3>(void *, const Geom &, int, int, const MetroParams &,
                                                const unsigned short *, const float *, unsigned long long *,
                                                hipStream_t);
-template void launch_metropolis_learned<double>(void *, const Geom &, int, int, const MetroParams &,
+template void launch_metropolis_learned<float, 2>(void *, const Geom &, int, int, const MetroParams &,
+                                               const unsigned short *, const float *, unsigned long long *,
+                                               hipStream_t);
+template void launch_metropolis_learned<double, 3>(void *, const Geom &, int, int, const MetroParams &,
                                                 const unsigned short *, const float *, unsigned long long *,
                                                 hipStream_t);
 

@@ -18,7 +18,7 @@
 
 namespace glacx {
 
-template <class T>
+template <class T, int R>
 __device__ __forceinline__ M3<T> clover(const typename vec2<T>::type *__restrict__ f, const Geom &g, const Coord &x,
                                         int mu, int nu, T &plaq)
 {
@@ -26,26 +26,26 @@ __device__ __forceinline__ M3<T> clover(const typename vec2<T>::type *__restrict
     const Coord xmm = step(g, x, mu, -1), xmn = step(g, x, nu, -1);
     const Coord xpmmn = step(g, xpm, nu, -1), xmmmn = step(g, xmm, nu, -1), xmmpn = step(g, xmm, nu, +1);
     const int ix = index_of(g, x), imn = index_of(g, xmn), imm = index_of(g, xmm);
-    const DirView<T> vmu = dir_view<T>(f, g.v, mu), vnu = dir_view<T>(f, g.v, nu);
-    const M3<T> umu = load_link<T>(vmu, ix);
-    const M3<T> unu = load_link<T>(vnu, ix);
+    const DirView<T> vmu = dir_view<T>(f, g, mu), vnu = dir_view<T>(f, g, nu);
+    const M3<T> umu = load_link<T, R>(vmu, ix);
+    const M3<T> unu = load_link<T, R>(vnu, ix);
     // L1 = U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
-    M3<T> c = mul_ad(mul_ad(mul(umu, load_link<T>(vnu, index_of(g, xpm))), load_link<T>(vmu, index_of(g, xpn))), unu);
+    M3<T> c = mul_ad(mul_ad(mul(umu, load_link<T, R>(vnu, index_of(g, xpm))), load_link<T, R>(vmu, index_of(g, xpn))), unu);
     plaq = retrace(c);
     // - L2 = -U_mu(x) U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu)
-    const M3<T> unu_mn = load_link<T>(vnu, imn);
+    const M3<T> unu_mn = load_link<T, R>(vnu, imn);
     {
-        M3<T> m = mul_ad(mul_ad(umu, load_link<T>(vnu, index_of(g, xpmmn))), load_link<T>(vmu, imn));
+        M3<T> m = mul_ad(mul_ad(umu, load_link<T, R>(vnu, index_of(g, xpmmn))), load_link<T, R>(vmu, imn));
         negate(m);
         mm<false, false, true>(c, m, unu_mn);
     }
     // + L3 = U_mu(x-mu)^dag U_nu(x-mu-nu)^dag U_mu(x-mu-nu) U_nu(x-nu)
-    const M3<T> umu_mm = load_link<T>(vmu, imm);
+    const M3<T> umu_mm = load_link<T, R>(vmu, imm);
     const int immmn = index_of(g, xmmmn);
-    mm<false, false, true>(c, mul(mul_dd(umu_mm, load_link<T>(vnu, immmn)), load_link<T>(vmu, immmn)), unu_mn);
+    mm<false, false, true>(c, mul(mul_dd(umu_mm, load_link<T, R>(vnu, immmn)), load_link<T, R>(vmu, immmn)), unu_mn);
     // - L4 = -U_mu(x-mu)^dag U_nu(x-mu) U_mu(x-mu+nu) U_nu(x)^dag
     {
-        M3<T> m = mul(mul_da(umu_mm, load_link<T>(vnu, imm)), load_link<T>(vmu, index_of(g, xmmpn)));
+        M3<T> m = mul(mul_da(umu_mm, load_link<T, R>(vnu, imm)), load_link<T, R>(vmu, index_of(g, xmmpn)));
         negate(m);
         mm<false, true, true>(c, m, unu);
     }
@@ -63,7 +63,49 @@ __device__ __forceinline__ M3<T> clover(const typename vec2<T>::type *__restrict
 constexpr int kMeasSites = 64;
 constexpr int kMeasWaves = 3;
 
-template <class T>
+// LDS exchange of the six field-strength tensors of a block's 64 sites
+// (SuperSampler variant): slot-major, then matrix entry, then site, so the
+// 64 lanes of a wavefront touch consecutive words.
+template <class T> struct MeasLds {
+    T v[6][18][kMeasSites];
+};
+template <class T> __device__ __forceinline__ MeasLds<T> &meas_lds()
+{
+    __shared__ MeasLds<T> s;
+    return s;
+}
+template <class T> __device__ __forceinline__ void lds_put(int slot, int lane, const M3<T> &m)
+{
+    MeasLds<T> &s = meas_lds<T>();
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+        s.v[slot][2 * e][lane] = m.e[e / 3][e % 3].r;
+        s.v[slot][2 * e + 1][lane] = m.e[e / 3][e % 3].i;
+    }
+}
+template <class T> __device__ __forceinline__ M3<T> lds_get(int slot, int lane)
+{
+    const MeasLds<T> &s = meas_lds<T>();
+    M3<T> m;
+#pragma unroll
+    for (int e = 0; e < 9; e++) m.e[e / 3][e % 3] = {s.v[slot][2 * e][lane], s.v[slot][2 * e + 1][lane]};
+    return m;
+}
+
+// Partials: kMeasAcc doubles per (chunk, t) slot and per time slice:
+// {plaq, E, q, w}; w (the Weinberg density) only in the SuperSampler variant.
+constexpr int kMeasAcc = 4;
+
+// SUPER adds GLAC's SuperSampler Weinberg operator
+// (GLAC/observables/supersampler.cpp:430-460): per site
+//   w = -sum_{nu=1..3} sum_{lambda != 0, nu} Re tr(G_{0 lambda} G_{nu lambda} G_{rho sigma}).
+// Wavefront nu holds G_{0 nu} and G_{rho sigma} for the block's sites; the
+// six tensors are exchanged through LDS (slot nu - 1: G_{0 nu}, slot 2 + nu:
+// G_{rho sigma}, i.e. G_23, G_31, G_12 in slots 3, 4, 5), then wavefront nu
+// forms its two terms with lambda_a = next(nu), lambda_b = next(lambda_a):
+// G_{nu lambda_a} is stored as such (slot 2 + lambda_b) and
+// G_{nu lambda_b} = G_{lambda_b nu}^dagger (slot 2 + lambda_a).
+template <class T, int R, bool SUPER>
 __global__ __launch_bounds__(kMeasSites *kMeasWaves) void measure_kernel(const typename vec2<T>::type *__restrict__ f,
                                                                          Geom g, int cps, double *__restrict__ partials)
 {
@@ -75,69 +117,89 @@ __global__ __launch_bounds__(kMeasSites *kMeasWaves) void measure_kernel(const t
     const int lane = threadIdx.x & 63;
     const int k = jc * (kMeasSites / 2) + (lane & 31);
     const int nu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) + 1;  // one clover pair per wavefront
-    double acc[3] = {0.0, 0.0, 0.0};
-    if (k < g.run) {
+    const bool valid = k < g.run;
+    double acc[kMeasAcc] = {0.0, 0.0, 0.0, 0.0};
+    M3<T> c2;
+    if (valid) {
         const Coord x = decode(g, run_index(g, blk, lane >> 5, lt, k));
         const int rho = nu % 3 + 1, sigma = rho % 3 + 1;
         T p1, p2;
-        const M3<T> c1 = traceless_antiherm2(clover<T>(f, g, x, 0, nu, p1));
-        const M3<T> c2 = traceless_antiherm2(clover<T>(f, g, x, rho, sigma, p2));
+        const M3<T> c1 = traceless_antiherm2(clover<T, R>(f, g, x, 0, nu, p1));
+        c2 = traceless_antiherm2(clover<T, R>(f, g, x, rho, sigma, p2));
         acc[0] = p1 + p2;
         acc[1] = retrace_mul(c1, c1) + retrace_mul(c2, c2);
         acc[2] = -retrace_mul(c1, c2);
+        if constexpr (SUPER) {
+            lds_put(nu - 1, lane, c1);
+            lds_put(2 + nu, lane, c2);
+        }
     }
-    block_sum<3, kMeasWaves>(acc);
+    if constexpr (SUPER) {
+        __syncthreads();
+        if (valid) {
+            const int la = nu % 3 + 1, lb = la % 3 + 1;
+            const double ta = retrace_mul(mul(lds_get<T>(la - 1, lane), lds_get<T>(2 + lb, lane)), c2);
+            const double tb = retrace_mul(mul_ad(lds_get<T>(lb - 1, lane), lds_get<T>(2 + la, lane)), c2);
+            acc[3] = -(ta + tb);
+        }
+    }
+    block_sum<kMeasAcc, kMeasWaves>(acc);
     if (threadIdx.x == 0) {
-        double *q = partials + 3 * (c * g.n[3] + t);
-        q[0] = acc[0];
-        q[1] = acc[1];
-        q[2] = acc[2];
+        double *q = partials + kMeasAcc * (c * g.n[3] + t);
+#pragma unroll
+        for (int i = 0; i < kMeasAcc; i++) q[i] = acc[i];
     }
 }
 
 // One block per time slice t: sums the partials of the chunks of slice t in
-// a fixed order into slice[3 t .. 3 t + 2].
+// a fixed order into slice[kMeasAcc t ...].
 __global__ __launch_bounds__(kBlock) void measure_slice_kernel(const double *__restrict__ partials, Geom g, int cps,
                                                                double *__restrict__ slice)
 {
     const int t = blockIdx.x;
-    double acc[3] = {0.0, 0.0, 0.0};
+    double acc[kMeasAcc] = {0.0, 0.0, 0.0, 0.0};
     for (int c = threadIdx.x; c < cps; c += kBlock) {
-        const double *q = partials + 3 * (c * g.n[3] + t);
-        acc[0] += q[0];
-        acc[1] += q[1];
-        acc[2] += q[2];
+        const double *q = partials + kMeasAcc * (c * g.n[3] + t);
+#pragma unroll
+        for (int i = 0; i < kMeasAcc; i++) acc[i] += q[i];
     }
-    block_sum<3>(acc);
-    if (threadIdx.x == 0) {
-        slice[3 * t + 0] = acc[0];
-        slice[3 * t + 1] = acc[1];
-        slice[3 * t + 2] = acc[2];
-    }
+    block_sum<kMeasAcc>(acc);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int i = 0; i < kMeasAcc; i++) slice[kMeasAcc * t + i] = acc[i];
 }
 
-// out = {plaq, E_raw, Q, Q(t = 0..Nt-1)}.
-__global__ void measure_finish_kernel(const double *__restrict__ slice, Geom g, double *__restrict__ out)
+// out = {plaq, E_raw, Q, Q(t = 0..Nt-1)}, or with `super`
+// {plaq, E_raw, Q, W, Q(t)..., W(t)...}; W's normalisation is GLAC's
+// 2 * 8 * 8 / (3 * 16^3) (GLAC/observables/supersampler.cpp:17-19).
+__global__ void measure_finish_kernel(const double *__restrict__ slice, Geom g, int super, double *__restrict__ out)
 {
     if (threadIdx.x != 0) return;
-    const double qnorm = 1.0 / (256.0 * M_PI * M_PI);
-    double pl = 0.0, en = 0.0, q = 0.0;
-    for (int t = 0; t < g.n[3]; t++) {
-        pl += slice[3 * t + 0];
-        en += slice[3 * t + 1];
-        const double qt = slice[3 * t + 2] * qnorm;
-        out[3 + t] = qt;
+    const double qnorm = 1.0 / (256.0 * M_PI * M_PI), wnorm = 2.0 * 8.0 * 8.0 / (3.0 * 16.0 * 16.0 * 16.0);
+    const int nt = g.n[3], head = super ? 4 : 3;
+    double pl = 0.0, en = 0.0, q = 0.0, w = 0.0;
+    for (int t = 0; t < nt; t++) {
+        pl += slice[kMeasAcc * t + 0];
+        en += slice[kMeasAcc * t + 1];
+        const double qt = slice[kMeasAcc * t + 2] * qnorm;
+        out[head + t] = qt;
         q += qt;
+        if (super) {
+            const double wt = slice[kMeasAcc * t + 3] * wnorm;
+            out[head + nt + t] = wt;
+            w += wt;
+        }
     }
     out[0] = pl / (18.0 * (double)g.v);
     out[1] = en / (double)g.v;
     out[2] = q;
+    if (super) out[3] = w;
 }
 
 // Plaquette only: sum over sites and the 6 planes of
 // Re tr[U_mu(x) U_nu(x+mu) (U_nu(x) U_mu(x+nu))^dagger]
 // (GLAC/observables/plaquette.cpp:26-40).
-template <class T>
+template <class T, int R>
 __global__ __launch_bounds__(kBlock) void plaquette_kernel(const typename vec2<T>::type *__restrict__ f, Geom g,
                                                            double *__restrict__ partials)
 {
@@ -148,12 +210,12 @@ __global__ __launch_bounds__(kBlock) void plaquette_kernel(const typename vec2<T
         T s = T(0);
 #pragma unroll 1
         for (int mu = 0; mu < 3; mu++) {
-            const M3<T> umu = load_link<T>(f, g.v, mu, idx);
+            const M3<T> umu = load_link<T, R>(f, g, mu, idx);
             const int ipm = index_of(g, step(g, x, mu, +1));
 #pragma unroll 1
             for (int nu = mu + 1; nu < 4; nu++) {
-                const M3<T> a = mul(umu, load_link<T>(f, g.v, nu, ipm));
-                const M3<T> b = mul(load_link<T>(f, g.v, nu, idx), load_link<T>(f, g.v, mu, index_of(g, step(g, x, nu, +1))));
+                const M3<T> a = mul(umu, load_link<T, R>(f, g, nu, ipm));
+                const M3<T> b = mul(load_link<T, R>(f, g, nu, idx), load_link<T, R>(f, g, mu, index_of(g, step(g, x, nu, +1))));
                 // Re tr(a b^dag) = sum_ij Re(a_ij conj(b_ij))
 #pragma unroll
                 for (int i = 0; i < 3; i++)
@@ -180,27 +242,30 @@ __global__ __launch_bounds__(kBlock) void plaquette_finish_kernel(const double *
 static inline int chunks_per_slab(const Geom &g) { return g.nzb * g.cpr; }
 
 // block partials followed by the per-slice sums
-int measure_partials_count(const Geom &g) { return 3 * g.n[3] * chunks_per_slab(g) + 3 * g.n[3]; }
+int measure_partials_count(const Geom &g) { return kMeasAcc * g.n[3] * chunks_per_slab(g) + kMeasAcc * g.n[3]; }
 int plaquette_partials_count(const Geom &g) { return (g.v + kBlock - 1) / kBlock; }
 
-template <class T> void launch_measure(const void *f, const Geom &g, double *partials, hipStream_t s)
+template <class T, int R> void launch_measure(const void *f, const Geom &g, double *partials, bool super, hipStream_t s)
 {
     const int cps = chunks_per_slab(g);
-    measure_kernel<T><<<g.n[3] * cps, kMeasSites * kMeasWaves, 0, s>>>((const typename vec2<T>::type *)f, g, cps,
-                                                                             partials);
+    const auto *fp = (const typename vec2<T>::type *)f;
+    if (super)
+        measure_kernel<T, R, true><<<g.n[3] * cps, kMeasSites * kMeasWaves, 0, s>>>(fp, g, cps, partials);
+    else
+        measure_kernel<T, R, false><<<g.n[3] * cps, kMeasSites * kMeasWaves, 0, s>>>(fp, g, cps, partials);
 }
 
-void launch_measure_finish(double *partials, const Geom &g, double *out, hipStream_t s)
+void launch_measure_finish(double *partials, const Geom &g, double *out, bool super, hipStream_t s)
 {
     const int cps = chunks_per_slab(g);
-    double *slice = partials + 3 * g.n[3] * cps;
+    double *slice = partials + kMeasAcc * g.n[3] * cps;
     measure_slice_kernel<<<g.n[3], kBlock, 0, s>>>(partials, g, cps, slice);
-    measure_finish_kernel<<<1, 64, 0, s>>>(slice, g, out);
+    measure_finish_kernel<<<1, 64, 0, s>>>(slice, g, super ? 1 : 0, out);
 }
 
-template <class T> void launch_plaquette(const void *f, const Geom &g, double *partials, hipStream_t s)
+template <class T, int R> void launch_plaquette(const void *f, const Geom &g, double *partials, hipStream_t s)
 {
-    plaquette_kernel<T><<<plaquette_partials_count(g), kBlock, 0, s>>>((const typename vec2<T>::type *)f, g, partials);
+    plaquette_kernel<T, R><<<plaquette_partials_count(g), kBlock, 0, s>>>((const typename vec2<T>::type *)f, g, partials);
 }
 
 void launch_plaquette_finish(const double *partials, int nparts, const Geom &g, double *out, hipStream_t s)
@@ -208,9 +273,11 @@ void launch_plaquette_finish(const double *partials, int nparts, const Geom &g, 
     plaquette_finish_kernel<<<1, kBlock, 0, s>>>(partials, nparts, g, out);
 }
 
-template void launch_measure<float>(const void *, const Geom &, double *, hipStream_t);
-template void launch_measure<double>(const void *, const Geom &, double *, hipStream_t);
-template void launch_plaquette<float>(const void *, const Geom &, double *, hipStream_t);
-template void launch_plaquette<double>(const void *, const Geom &, double *, hipStream_t);
+template void launch_measure<float, 3>(const void *, const Geom &, double *, bool, hipStream_t);
+template void launch_measure<float, 2>(const void *, const Geom &, double *, bool, hipStream_t);
+template void launch_measure<double, 3>(const void *, const Geom &, double *, bool, hipStream_t);
+template void launch_plaquette<float, 3>(const void *, const Geom &, double *, hipStream_t);
+template void launch_plaquette<float, 2>(const void *, const Geom &, double *, hipStream_t);
+template void launch_plaquette<double, 3>(const void *, const Geom &, double *, hipStream_t);
 
 } // namespace glacx

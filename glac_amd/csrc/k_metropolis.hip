@@ -23,7 +23,7 @@
 
 namespace glacx {
 
-template <class T>
+template <class T, int R>
 __global__ __launch_bounds__(kBlock) void metropolis_kernel(typename vec2<T>::type *__restrict__ f, Geom g, int mu,
                                                             int parity, MetroParams prm,
                                                             unsigned long long *__restrict__ count)
@@ -33,8 +33,8 @@ __global__ __launch_bounds__(kBlock) void metropolis_kernel(typename vec2<T>::ty
     if (h < g.vh) {
         const int idx = parity_index(g, parity, h);
         const Coord x = decode(g, idx);
-        const M3<T> a = staple<T>(f, g, x, mu);
-        M3<T> u = load_link<T>(f, g.v, mu, idx);
+        const M3<T> a = staple<T, R>(f, g, x, mu);
+        M3<T> u = load_link<T, R>(f, g, mu, idx);
         Stream rs(prm.seed, (uint32_t)lex_of(g, x), (uint32_t)mu, STREAM_METROPOLIS, prm.sweep);
         const T eps = T(prm.eps);
         const T x0abs = sqrt(T(1) - eps * eps);
@@ -58,24 +58,26 @@ __global__ __launch_bounds__(kBlock) void metropolis_kernel(typename vec2<T>::ty
             reunitarize(d);
             u = convert<T>(d);
         }
-        store_link<T>(f, g.v, mu, idx, u);
+        store_link<T, R>(f, g, mu, idx, u);
         nacc[0] = (double)accepted;
     }
     block_sum<1>(nacc);
     if (threadIdx.x == 0) atomicAdd(count, (unsigned long long)nacc[0]);
 }
 
-template <class T>
+template <class T, int R>
 void launch_metropolis(void *f, const Geom &g, int mu, int parity, const MetroParams &p, unsigned long long *count,
                        hipStream_t s)
 {
     const unsigned blocks = (unsigned)((g.vh + kBlock - 1) / kBlock);
-    metropolis_kernel<T><<<blocks, kBlock, 0, s>>>((typename vec2<T>::type *)f, g, mu, parity, p, count);
+    metropolis_kernel<T, R><<<blocks, kBlock, 0, s>>>((typename vec2<T>::type *)f, g, mu, parity, p, count);
 }
 
-template void launch_metropolis<float>(void *, const Geom &, int, int, const MetroParams &, unsigned long long *,
+template void launch_metropolis<float, 3>(void *, const Geom &, int, int, const MetroParams &, unsigned long long *,
                                        hipStream_t);
-template void launch_metropolis<double>(void *, const Geom &, int, int, const MetroParams &, unsigned long long *,
+template void launch_metropolis<float, 2>(void *, const Geom &, int, int, const MetroParams &, unsigned long long *,
+                                       hipStream_t);
+template void launch_metropolis<double, 3>(void *, const Geom &, int, int, const MetroParams &, unsigned long long *,
                                         hipStream_t);
 
 } // namespace glacx

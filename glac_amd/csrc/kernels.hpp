@@ -54,37 +54,38 @@ template <int N, int WAVES = kBlock / 64> __device__ __forceinline__ void block_
 }
 
 // ---- launchers (explicitly instantiated for float and double) ------------
-template <class T> void launch_pack(const double *stage, void *f, const Geom &g, int site_major, hipStream_t s);
-template <class T> void launch_unpack(const void *f, double *stage, const Geom &g, int site_major, hipStream_t s);
-template <class T> void launch_hot_start(void *f, const Geom &g, uint64_t seed, hipStream_t s);
-template <class T> void launch_cold_start(void *f, const Geom &g, hipStream_t s);
-template <class T>
+template <class T, int R> void launch_pack(const double *stage, void *f, const Geom &g, int site_major, hipStream_t s);
+template <class T, int R> void launch_unpack(const void *f, double *stage, const Geom &g, int site_major, hipStream_t s);
+template <class T, int R> void launch_hot_start(void *f, const Geom &g, uint64_t seed, hipStream_t s);
+template <class T, int R> void launch_cold_start(void *f, const Geom &g, hipStream_t s);
+template <class T, int R>
 void launch_action_derivative(const void *f, double *stage, const Geom &g, int mu, hipStream_t s);
 
 // Clover measurement: per-block partial sums, then a one-block fixed-order
-// reduction into out[0..2+Nt] = {plaq, E_raw, Q, Q(t)...}.
+// reduction into out = {plaq, E_raw, Q, Q(t)...}, or with `super` (GLAC's
+// SuperSampler) {plaq, E_raw, Q, W, Q(t)..., W(t)...}.
 int measure_partials_count(const Geom &g);
-template <class T> void launch_measure(const void *f, const Geom &g, double *partials, hipStream_t s);
-void launch_measure_finish(double *partials, const Geom &g, double *out, hipStream_t s);
+template <class T, int R> void launch_measure(const void *f, const Geom &g, double *partials, bool super, hipStream_t s);
+void launch_measure_finish(double *partials, const Geom &g, double *out, bool super, hipStream_t s);
 
 int plaquette_partials_count(const Geom &g);
-template <class T> void launch_plaquette(const void *f, const Geom &g, double *partials, hipStream_t s);
+template <class T, int R> void launch_plaquette(const void *f, const Geom &g, double *partials, hipStream_t s);
 void launch_plaquette_finish(const double *partials, int nparts, const Geom &g, double *out, hipStream_t s);
 
 // One RK3 stage (0, 1, 2): win -> wout, accumulator read/written in acc.
-template <class T>
+template <class T, int R>
 void launch_flow_stage(const void *win, void *wout, void *acc, const Geom &g, int stage, double eps, hipStream_t s);
 
 // One checkerboard sub-sweep (mu, parity), in place; accepted hits are added
 // to *count.
-template <class T>
+template <class T, int R>
 void launch_metropolis(void *f, const Geom &g, int mu, int parity, const MetroParams &p,
                        unsigned long long *count, hipStream_t s);
 
 // ---- learned proposal (k_learned.hip) ----------------------------------------
 // Samples are stored column-major: column c (0..35: A then U, 18 reals each)
 // of sample s at out[c * cap + s].
-template <class T>
+template <class T, int R>
 void launch_collect(const void *f, const Geom &g, int mu, int parity, int stride, float *out, size_t cap, size_t base,
                     hipStream_t s);
 int feature_moments_blocks(size_t n);
@@ -98,7 +99,7 @@ void launch_train_step(const float *smp, size_t cap, size_t n_samples, int batch
                        float *inter, double *loss_partials, float *grad, float *m, float *v, float lr, int adam_t,
                        unsigned short *bf16, hipStream_t s);
 void launch_to_bf16(const float *params, unsigned short *bf16, hipStream_t s);
-template <class T>
+template <class T, int R>
 void launch_metropolis_learned(void *f, const Geom &g, int mu, int parity, const MetroParams &p,
                                const unsigned short *wbf16, const float *mean_std, unsigned long long *count,
                                hipStream_t s);

@@ -207,6 +207,13 @@ template <> struct vec2<double> {
 // buffer_load_dwordx2/x4 with the per-lane byte offset idx * sizeof(V2) in a
 // VGPR and the plane offset k V sizeof(V2) in an SGPR -- no per-load address
 // arithmetic.  The context limits 9 V sizeof(V2) to below 2^32 bytes.
+//
+// Stored rows R (a template parameter of every kernel that touches links):
+// R = 3 stores the whole matrix; R = 2 (the compressed FP32 mode) stores and
+// reads only the first two rows and rebuilds the third as
+// U_2j = conj(U_0k U_1l - U_0l U_1k), (j, k, l) cyclic, which holds for every
+// SU(3) matrix (orthonormal rows, determinant 1).  A third of the link bytes
+// is then never moved.
 constexpr int kBufferFlags = 0x00020000;  // gfx9 raw buffer: 32-bit data format, no swizzle
 
 template <class T> struct DirView {
@@ -215,13 +222,13 @@ template <class T> struct DirView {
 };
 
 template <class T>
-__device__ __forceinline__ DirView<T> dir_view(const typename vec2<T>::type *f, int v, int mu)
+__device__ __forceinline__ DirView<T> dir_view(const typename vec2<T>::type *f, const Geom &g, int mu)
 {
     using V2 = typename vec2<T>::type;
     DirView<T> d;
-    const unsigned range = 9u * (unsigned)v * (unsigned)sizeof(V2);
-    d.r = __builtin_amdgcn_make_buffer_rsrc((void *)(f + (size_t)mu * 9 * v), (short)0, (int)range, kBufferFlags);
-    d.plane_bytes = v * (int)sizeof(V2);
+    const unsigned range = 9u * (unsigned)g.v * (unsigned)sizeof(V2);
+    d.r = __builtin_amdgcn_make_buffer_rsrc((void *)(f + (size_t)mu * 9 * g.v), (short)0, (int)range, kBufferFlags);
+    d.plane_bytes = g.v * (int)sizeof(V2);
     return d;
 }
 
@@ -244,25 +251,48 @@ __device__ __forceinline__ void buf_store(const DirView<double> &d, unsigned off
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, z), d.r, off, k * d.plane_bytes, 0);
 }
 
-template <class T> __device__ __forceinline__ M3<T> load_link(const DirView<T> &d, int idx)
+// Third row of an SU(3) matrix from its first two.
+template <class T> __host__ __device__ __forceinline__ void rebuild_third_row(M3<T> &m)
+{
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int k = (j + 1) % 3, l = (j + 2) % 3;
+        const cpx<T> a = m.e[0][k], b = m.e[1][l], c = m.e[0][l], d = m.e[1][k];
+        const T re = fma(a.r, b.r, fma(-a.i, b.i, fma(-c.r, d.r, c.i * d.i)));
+        const T im = fma(a.r, b.i, fma(a.i, b.r, fma(-c.r, d.i, -(c.i * d.r))));
+        m.e[2][j] = {re, -im};
+    }
+}
+
+template <class T, int R> __device__ __forceinline__ M3<T> load_link(const DirView<T> &d, int idx)
 {
     using V2 = typename vec2<T>::type;
     const unsigned off = (unsigned)idx * (unsigned)sizeof(V2);
     M3<T> m;
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
+    for (int k = 0; k < 6; k++) {
         const V2 z = buf_load(d, off, k);
         m.e[k / 3][k % 3] = {z.x, z.y};
+    }
+    static_assert(R == 2 || R == 3, "links store 2 or 3 rows");
+    if (R == 2) {
+        rebuild_third_row(m);
+    } else {
+#pragma unroll
+        for (int k = 6; k < 9; k++) {
+            const V2 z = buf_load(d, off, k);
+            m.e[2][k - 6] = {z.x, z.y};
+        }
     }
     return m;
 }
 
-template <class T> __device__ __forceinline__ void store_link(const DirView<T> &d, int idx, const M3<T> &m)
+template <class T, int R> __device__ __forceinline__ void store_link(const DirView<T> &d, int idx, const M3<T> &m)
 {
     using V2 = typename vec2<T>::type;
     const unsigned off = (unsigned)idx * (unsigned)sizeof(V2);
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
+    for (int k = 0; k < 3 * R; k++) {
         V2 z;
         z.x = m.e[k / 3][k % 3].r;
         z.y = m.e[k / 3][k % 3].i;
@@ -271,60 +301,60 @@ template <class T> __device__ __forceinline__ void store_link(const DirView<T> &
 }
 
 // Convenience forms for a wave-uniform mu.
-template <class T>
-__device__ __forceinline__ M3<T> load_link(const typename vec2<T>::type *f, int v, int mu, int idx)
+template <class T, int R>
+__device__ __forceinline__ M3<T> load_link(const typename vec2<T>::type *f, const Geom &g, int mu, int idx)
 {
-    return load_link<T>(dir_view<T>(f, v, mu), idx);
+    return load_link<T, R>(dir_view<T>(f, g, mu), idx);
 }
-template <class T>
-__device__ __forceinline__ void store_link(typename vec2<T>::type *f, int v, int mu, int idx, const M3<T> &m)
+template <class T, int R>
+__device__ __forceinline__ void store_link(typename vec2<T>::type *f, const Geom &g, int mu, int idx, const M3<T> &m)
 {
-    store_link<T>(dir_view<T>(f, v, mu), idx, m);
+    store_link<T, R>(dir_view<T>(f, g, mu), idx, m);
 }
 
-// Plain-pointer forms for kernels in which mu varies within a wavefront.
-template <class T>
-__device__ __forceinline__ void store_link_any(typename vec2<T>::type *__restrict__ f, int v, int mu, int idx,
+// Plain-pointer form for kernels in which mu varies within a wavefront.
+template <class T, int R>
+__device__ __forceinline__ void store_link_any(typename vec2<T>::type *__restrict__ f, const Geom &g, int mu, int idx,
                                                const M3<T> &m)
 {
-    typename vec2<T>::type *p = f + (size_t)mu * 9 * v + idx;
+    typename vec2<T>::type *p = f + (size_t)mu * 9 * g.v + idx;
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
+    for (int k = 0; k < 3 * R; k++) {
         typename vec2<T>::type z;
         z.x = m.e[k / 3][k % 3].r;
         z.y = m.e[k / 3][k % 3].i;
-        p[(size_t)k * v] = z;
+        p[(size_t)k * g.v] = z;
     }
 }
 
 // Wilson staple (mu wave-uniform) A_mu(x) = sum_{nu != mu} [ U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
 //                                        + U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu) ]
 // (GLAC/actions/wilsongaugeaction.cpp:29-53).
-template <class T>
+template <class T, int R>
 __device__ __forceinline__ M3<T> staple(const typename vec2<T>::type *__restrict__ f, const Geom &g, const Coord &x,
                                         int mu)
 {
     M3<T> a = zero3<T>();
     const Coord xpm = step(g, x, mu, +1);
     const int ix = index_of(g, x), ipm = index_of(g, xpm);
-    const DirView<T> vmu = dir_view<T>(f, g.v, mu);
+    const DirView<T> vmu = dir_view<T>(f, g, mu);
 #pragma unroll 1
     for (int d = 1; d < 4; d++) {
         const int nu = (mu + d) & 3;
-        const DirView<T> vnu = dir_view<T>(f, g.v, nu);
+        const DirView<T> vnu = dir_view<T>(f, g, nu);
         const int ipn = index_of(g, step(g, x, nu, +1));
         const int imn = index_of(g, step(g, x, nu, -1));
         const int ipmmn = index_of(g, step(g, xpm, nu, -1));
         {
-            const M3<T> u1 = load_link<T>(vnu, ipm);
-            const M3<T> u2 = load_link<T>(vmu, ipn);
-            const M3<T> u3 = load_link<T>(vnu, ix);
+            const M3<T> u1 = load_link<T, R>(vnu, ipm);
+            const M3<T> u2 = load_link<T, R>(vmu, ipn);
+            const M3<T> u3 = load_link<T, R>(vnu, ix);
             mm<false, true, true>(a, mul_ad(u1, u2), u3);
         }
         {
-            const M3<T> u1 = load_link<T>(vnu, ipmmn);
-            const M3<T> u2 = load_link<T>(vmu, imn);
-            const M3<T> u3 = load_link<T>(vnu, imn);
+            const M3<T> u1 = load_link<T, R>(vnu, ipmmn);
+            const M3<T> u2 = load_link<T, R>(vmu, imn);
+            const M3<T> u3 = load_link<T, R>(vnu, imn);
             mm<false, false, true>(a, mul_dd(u1, u2), u3);
         }
     }

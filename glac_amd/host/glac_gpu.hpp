@@ -95,6 +95,18 @@ public:
         }
     }
 
+    // Read a configuration file straight into the device copy of `lattice`
+    // (GLAC's .bin, or Chroma's big-endian variant); the host copy is then
+    // stale until syncHost().
+    template <class L> void loadInto(L *lattice, const std::string &path, bool chroma)
+    {
+        Binding &b = bind(lattice);
+        check(chroma ? glacx_load_chroma(m_ctx, b.field, path.c_str()) : glacx_load_bin(m_ctx, b.field, path.c_str()),
+              chroma ? "load_chroma" : "load_bin");
+        b.host_newer = false;
+        b.device_newer = true;
+    }
+
 private:
     struct Binding {
         const void *host = nullptr;
@@ -288,6 +300,27 @@ private:
     Device *m_dev;
     int m_nt;
     std::vector<std::vector<double>> m_obs;
+};
+
+// ----------------------------------------------------------------- field IO ---
+// IO::FieldIO (GLAC/io/fieldio.cpp:61-122, 195-248, 259-338) on device-resident
+// lattices: loads go to the device copy, writes read the device copy, in
+// GLAC's .bin layout (site-major, 4 links per site, 18 doubles per link).
+// Paths are complete (GLAC prefixes Parameters::getFilePath() and the
+// input / output folders itself).
+template <class L> class FieldIO {
+public:
+    explicit FieldIO(Device *device) : m_dev(device) {}
+    void loadFieldConfiguration(const std::string &path, L *lattice) { m_dev->loadInto(lattice, path, false); }
+    void loadChromaFieldConfiguration(const std::string &path, L *lattice) { m_dev->loadInto(lattice, path, true); }
+    void writeFieldToFile(const std::string &path, L *lattice)
+    {
+        const int f = m_dev->acquire(lattice);
+        m_dev->check(glacx_save_bin(m_dev->ctx(), f, path.c_str()), "save_bin");
+    }
+
+private:
+    Device *m_dev;
 };
 
 // --------------------------------------------------------------- Metropolis ---

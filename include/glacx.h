@@ -33,8 +33,12 @@ typedef struct glacx_ctx glacx_ctx;
 
 enum glacx_precision {
     GLACX_FP64 = 0,           /* links, staples, exponential and sums in FP64 */
-    GLACX_FP32_LINKS = 1      /* FP32 links/staples/flow; FP64 exp coefficients,
+    GLACX_FP32_LINKS = 1,     /* FP32 links/staples/flow; FP64 exp coefficients,
                                  reunitarisation and lattice-wide sums */
+    GLACX_FP32_COMPRESSED = 2 /* as GLACX_FP32_LINKS, but only the first two rows
+                                 of each link are stored; the third is rebuilt
+                                 as conj(row0 x row1).  SU(3) links only:
+                                 uploads and loads of other links fail */
 };
 
 enum glacx_field { GLACX_FIELD_MAIN = 0, GLACX_FIELD_FLOW = 1 };
@@ -81,6 +85,10 @@ int glacx_copy_field(glacx_ctx *ctx, int dst, int src);
  * per link, no header; 576 V bytes. */
 int glacx_load_bin(glacx_ctx *ctx, int field, const char *path);
 int glacx_save_bin(glacx_ctx *ctx, int field, const char *path);
+/* Chroma configuration (IO::FieldIO::loadChromaFieldConfiguration,
+ * GLAC/io/fieldio.cpp:259-338): the same site and link order as .bin, every
+ * double big-endian. */
+int glacx_load_chroma(glacx_ctx *ctx, int field, const char *path);
 
 /* Starts (GLAC/system.cpp:200-224).  Hot start draws Haar-random SU(3) links
  * from the counter-based stream (seed, site, mu); cold start sets unity. */
@@ -108,6 +116,13 @@ int glacx_metropolis(glacx_ctx *ctx, int n_sweeps, double beta, double eps, int 
  * q_t[Nt], the charge summed over each time slice.  Any pointer may be NULL. */
 int glacx_measure(glacx_ctx *ctx, int field, double *plaq, double *e_raw, double *q, double *q_t);
 
+/* glacx_measure plus the Weinberg operator, as GLAC's SuperSampler
+ * (GLAC/observables/supersampler.cpp:298-496): w = W and w_t[Nt] its
+ * per-time-slice sums, normalised by GLAC's 2 * 8 * 8 / (3 * 16^3).  Any
+ * pointer may be NULL. */
+int glacx_measure_super(glacx_ctx *ctx, int field, double *plaq, double *e_raw, double *q, double *w, double *q_t,
+                        double *w_t);
+
 /* Plaquette only (Plaquette::calculate, GLAC/observables/plaquette.cpp:26-40). */
 int glacx_plaquette(glacx_ctx *ctx, int field, double *plaq);
 
@@ -120,6 +135,12 @@ int glacx_plaquette(glacx_ctx *ctx, int field, double *plaq);
  * q_t holds (n_steps + 1) * Nt values, row i = flow time i * eps. */
 int glacx_flow(glacx_ctx *ctx, int field, int n_steps, double eps,
                double *plaq, double *e_raw, double *q, double *q_t);
+
+/* glacx_flow measuring with the SuperSampler observables (GLAC's flow with
+ * "weinberg" in the observable list, GLAC/system.cpp:76-90): w holds n_steps + 1
+ * values and w_t (n_steps + 1) * Nt, laid out as q and q_t. */
+int glacx_flow_super(glacx_ctx *ctx, int field, int n_steps, double eps, double *plaq, double *e_raw, double *q,
+                     double *w, double *q_t, double *w_t);
 
 /* Whole-lattice flow force for one direction in GLAC's layout, V*18 doubles
  * (WilsonGaugeAction::getActionDerivative,

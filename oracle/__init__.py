@@ -65,6 +65,7 @@ def lib():
         L.or_su3_exp_taylor.argtypes = [_dp, C.c_int, _dp]
         L.or_flow_step.argtypes = [_dp, _i4, C.c_double]
         L.or_measure.argtypes = [_dp, _i4, _dp, _dp]
+        L.or_measure_super.argtypes = [_dp, _i4, _dp, _dp, _dp]
         L.or_glac_rng_new.argtypes = [C.c_uint64, C.c_uint64]
         L.or_glac_rng_new.restype = C.c_void_p
         L.or_glac_rng_free.argtypes = [C.c_void_p]
@@ -245,6 +246,15 @@ def measure(U, n):
     qt = np.empty(int(n[3]))
     lib().or_measure(np.ascontiguousarray(U), dims4(n), out, qt)
     return float(out[0]), float(out[1]), float(out[2]), qt
+
+
+def measure_super(U, n):
+    """Return (plaquette, E_raw, Q, W, Q(t), W(t)) as GLAC's SuperSampler."""
+    out = np.empty(4)
+    qt = np.empty(int(n[3]))
+    wt = np.empty(int(n[3]))
+    lib().or_measure_super(np.ascontiguousarray(U), dims4(n), out, qt, wt)
+    return float(out[0]), float(out[1]), float(out[2]), float(out[3]), qt, wt
 
 
 # ------------------------------------------------------------- Metropolis

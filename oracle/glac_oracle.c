@@ -51,6 +51,15 @@
  *      Pinned: unit field (plaq = 1, E = Q = 0); constant abelian fields
  *      with closed-form plaquette and energy and integer charge Q; gauge
  *      invariance; sum_t Q(t) = Q; parity flips Q.
+ *  or_measure_super   GLAC/observables/supersampler.cpp:298-496 (adds the
+ *      Weinberg operator W and W(t) to or_measure's observables).  Pinned:
+ *      plaq, E, Q, Q(t) equal or_measure's; W = 0 on commuting (abelian)
+ *      fields with flux in three planes sharing a direction, where a wrong
+ *      orientation of one G_{nu lambda} term would leave a non-zero
+ *      commutator trace; gauge invariance; time reflection flips W;
+ *      sum_t W(t) = W.  The overall scale 2 * 8 * 8 / (3 * 16^3) is GLAC's
+ *      constant (supersampler.cpp:19) and parity unpinned beyond that
+ *      reading: GLAC ships no Weinberg reference values.
  *  or_glac_hot_start / or_glac_sweep   GLAC/system.cpp:200-214, 321-361.
  *      Pinned statistically: strong-coupling plaquette beta/18, acceptance
  *      near GLAC's recorded 0.108 at 4^3x8, beta 6, eps 0.24, 10 hits
@@ -657,6 +666,63 @@ void or_measure(const double *U, const int n[4], double out[3], double *qt)
     out[0] = plaq / (18.0 * (double)V);
     out[1] = energy / (double)V;
     out[2] = q;
+}
+
+/* GLAC's SuperSampler: the clover observables plus the Weinberg operator
+ * W = -(2 * 8 * 8 / (3 * 16^3)) sum_x sum_{nu=1..3} sum_{lambda != 0, nu}
+ *       Re tr(G_{0 lambda} G_{nu lambda} G_{rho sigma}),
+ * (nu, rho, sigma) cyclic in (1, 2, 3), G = traceless(C - C^dag) of the
+ * clover, G_{nu lambda} = G_{lambda nu}^dag for the orientations GLAC does
+ * not store (supersampler.cpp:430-460, index map supersampler.h:65-70).
+ * out = {plaq, E_raw, Q, W}; qt[Nt], wt[Nt] per time slice. */
+void or_measure_super(const double *U, const int n[4], double out[4], double *qt, double *wt)
+{
+    long V = volume(n);
+    double plaq = 0.0, energy = 0.0;
+    for (int t = 0; t < n[3]; t++) qt[t] = wt[t] = 0.0;
+    for (long s = 0; s < V; s++) {
+        int c[4];
+        decode(n, s, c);
+        double G[4][4][18]; /* G[a][b] for a < b as computed, other order by dagger */
+        double qs = 0.0, ws = 0.0;
+        for (int nu = 1; nu < 4; nu++) {
+            int rho = nu % 3 + 1, sigma = rho % 3 + 1;
+            double C1[18], C2[18];
+            plaq += clover(U, n, c, 0, nu, C1);
+            plaq += clover(U, n, c, rho, sigma, C2);
+            traceless_antiherm(C1);
+            traceless_antiherm(C2);
+            qs -= or_su3_retrace_mul(C1, C2);
+            energy += or_su3_retrace_mul(C1, C1) + or_su3_retrace_mul(C2, C2);
+            memcpy(G[0][nu], C1, sizeof C1);
+            memcpy(G[rho][sigma], C2, sizeof C2);
+            or_su3_dag(C1, G[nu][0]);
+            or_su3_dag(C2, G[sigma][rho]);
+        }
+        for (int nu = 1; nu < 4; nu++) {
+            int rho = nu % 3 + 1, sigma = rho % 3 + 1;
+            for (int lam = 1; lam < 4; lam++) {
+                if (lam == nu) continue;
+                double T[18];
+                or_su3_mul(G[0][lam], G[nu][lam], T);
+                ws -= or_su3_retrace_mul(T, G[rho][sigma]);
+            }
+        }
+        qt[c[3]] += qs;
+        wt[c[3]] += ws;
+    }
+    const double qnorm = 1.0 / (256.0 * M_PI * M_PI), wnorm = 2.0 * 8.0 * 8.0 / (3.0 * 16.0 * 16.0 * 16.0);
+    double q = 0.0, w = 0.0;
+    for (int t = 0; t < n[3]; t++) {
+        qt[t] *= qnorm;
+        wt[t] *= wnorm;
+        q += qt[t];
+        w += wt[t];
+    }
+    out[0] = plaq / (18.0 * (double)V);
+    out[1] = energy / (double)V;
+    out[2] = q;
+    out[3] = w;
 }
 
 /* ======================================================================== */

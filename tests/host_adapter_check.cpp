@@ -8,10 +8,13 @@
 //                            updates through the adapter, checked against
 //                            the FP64 oracle; device residency is checked by
 //                            counting host<->device transfers.
+//   host_adapter_check gpu <dir>   also writes/reads configurations in <dir>
+//                            through FieldIO (.bin and Chroma).
 // Prints "PASS ..." lines; exits non-zero on the first failure.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "../glac_amd/host/glac_gpu.hpp"
@@ -75,7 +78,7 @@ static int run_cpu()
     return 0;
 }
 
-static int run_gpu()
+static int run_gpu(const std::string &dir)
 {
     const std::vector<unsigned int> N = {4, 4, 4, 8};
     const int n[4] = {4, 4, 4, 8};
@@ -152,6 +155,39 @@ static int run_gpu()
         }
     REQUIRE(dmax < 1e-12, "det deviation %g", dmax);
     std::printf("PASS Metropolis update, acceptance %.4f\n", acc);
+
+    // FieldIO: write the chain through the device, read it back into another
+    // host lattice, then the same data as a big-endian Chroma file
+    const std::string bin = dir + "/adapter_cfg.bin", chroma = dir + "/adapter_cfg.chroma";
+    glac_gpu::FieldIO<Lattice<SU3>> fio(&dev2);
+    fio.writeFieldToFile(bin, M);
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1) {
+            FILE *in = std::fopen(bin.c_str(), "rb");
+            FILE *out = std::fopen(chroma.c_str(), "wb");
+            REQUIRE(in && out, "cannot open %s / %s", bin.c_str(), chroma.c_str());
+            unsigned char b[8], r[8];
+            while (std::fread(b, 1, 8, in) == 8) {
+                for (int k = 0; k < 8; k++) r[k] = b[7 - k];
+                std::fwrite(r, 1, 8, out);
+            }
+            std::fclose(in);
+            std::fclose(out);
+        }
+        Lattice<SU3> R[4];
+        for (int mu = 0; mu < 4; mu++) R[mu].allocate(N);
+        if (pass == 0)
+            fio.loadFieldConfiguration(bin, R);
+        else
+            fio.loadChromaFieldConfiguration(chroma, R);
+        dev2.syncHost(R);
+        double e = 0;
+        for (int mu = 0; mu < 4; mu++)
+            for (unsigned long s = 0; s < V; s++)
+                for (int k = 0; k < 18; k++) e = std::fmax(e, std::fabs(R[mu].m_sites[s].mat[k] - M[mu].m_sites[s].mat[k]));
+        REQUIRE(e == 0.0, "%s round trip differs by %g", pass ? "chroma" : "bin", e);
+    }
+    std::printf("PASS FieldIO .bin and Chroma round trips\n");
     return 0;
 }
 
@@ -160,6 +196,6 @@ int main(int argc, char **argv)
     if (argc < 2) return 2;
     const std::string mode = argv[1];
     if (mode == "cpu") return run_cpu();
-    if (mode == "gpu") return run_gpu();
+    if (mode == "gpu") return run_gpu(argc > 2 ? argv[2] : "/tmp");
     return 2;
 }

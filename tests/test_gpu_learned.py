@@ -57,7 +57,7 @@ def test_train_gradient_matches_reference():
     assert np.abs(g - want_g).max() < 2e-3 * scale, np.abs(g - want_g).max() / scale
 
 
-@pytest.mark.parametrize("prec", [glacx.FP64, glacx.FP32_LINKS], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("prec", [glacx.FP64, glacx.FP32_LINKS, glacx.FP32_COMPRESSED], ids=["fp64", "fp32", "fp32c"])
 def test_learned_sweep_follows_oracle_streams(prec):
     n = (4, 4, 4, 8)
     banner(f"learned sweep vs oracle {n} prec={prec}")

@@ -18,8 +18,8 @@ pytestmark = pytest.mark.gpu
 
 glacx = pytest.importorskip("glac_amd")
 
-PRECISIONS = [glacx.FP64, glacx.FP32_LINKS]
-PNAME = {glacx.FP64: "fp64", glacx.FP32_LINKS: "fp32"}
+PRECISIONS = [glacx.FP64, glacx.FP32_LINKS, glacx.FP32_COMPRESSED]
+PNAME = {glacx.FP64: "fp64", glacx.FP32_LINKS: "fp32", glacx.FP32_COMPRESSED: "fp32c"}
 
 
 def banner(msg):
@@ -237,6 +237,24 @@ def test_metropolis_reproduces_glac_recorded_run(prec):
     assert np.abs(np.linalg.det(M) - 1).max() < tol * 10
 
 
+def test_chroma_load_is_big_endian_bin(tmp_path, prec):
+    n = (4, 4, 4, 6)
+    banner(f"chroma load {n} {PNAME[prec]}")
+    U = random_field(n, np.random.default_rng(7))
+    site_major = np.ascontiguousarray(U.reshape(4, -1, 18).transpose(1, 0, 2))
+    path = str(tmp_path / "cfg.chroma")
+    site_major.astype(">f8").tofile(path)
+    with glacx.Lattice(n, prec) as L:
+        L.load_chroma(path)
+        got = L.download()
+        p_chroma = L.plaquette()
+        L.upload(U)
+        p_direct = L.plaquette()
+    tol = 0 if prec == glacx.FP64 else 1e-6
+    assert np.abs(got - U).max() <= tol
+    assert abs(p_chroma - p_direct) <= tol
+
+
 def test_bin_roundtrip_glac_layout(tmp_path, prec):
     n = (4, 4, 6, 8)
     banner(f"bin io {n} {PNAME[prec]}")
@@ -255,6 +273,69 @@ def test_bin_roundtrip_glac_layout(tmp_path, prec):
     tol = 0.0 if prec == glacx.FP64 else 1e-7
     assert np.abs(site_major - U).max() <= tol
     assert np.abs(back - U).max() <= tol
+
+
+def test_super_sampler_weinberg_matches_oracle(prec):
+    n = (4, 6, 4, 8)
+    banner(f"super sampler {n} {PNAME[prec]}")
+    U = near_unit_field(n, np.random.default_rng(21), 0.6)
+    want = O.measure_super(U, n)
+    with glacx.Lattice(n, prec) as L:
+        L.upload(U)
+        got = L.measure_super()
+        plain = L.measure()
+    assert abs(want[3]) > 1e-4            # the Weinberg value is not trivially zero
+    if prec == glacx.FP64:
+        for i in range(4):
+            assert abs(got[i] - want[i]) < 1e-11 * max(1.0, abs(want[i])), i
+        assert np.abs(got[4] - want[4]).max() < 1e-12 and np.abs(got[5] - want[5]).max() < 1e-12
+    else:
+        assert abs(got[0] - want[0]) < 5e-7 and abs(got[1] / want[1] - 1) < 1e-5
+        assert abs(got[2] - want[2]) < 1e-4 and abs(got[3] - want[3]) < 1e-4 * max(1.0, abs(want[3]))
+        assert np.abs(got[5] - want[5]).max() < 1e-4 * max(1.0, np.abs(want[5]).max())
+    # the plain measurement agrees with the SuperSampler's shared observables
+    assert plain[0] == got[0] and plain[1] == got[1] and plain[2] == got[2]
+    assert abs(got[5].sum() - got[3]) < 1e-12 * max(1.0, abs(got[3]))
+
+
+def test_flow_super_matches_oracle(prec):
+    n = (4, 4, 4, 6)
+    banner(f"flow super {n} {PNAME[prec]}")
+    U = near_unit_field(n, np.random.default_rng(22), 0.5)
+    W = U.copy()
+    rows = [O.measure_super(W, n)]
+    for _ in range(3):
+        O.flow_step(W, n, 0.02)
+        rows.append(O.measure_super(W, n))
+    with glacx.Lattice(n, prec) as L:
+        L.upload(U, field=glacx.FLOW)
+        p, e, q, w, qt, wt = L.flow_super(3, 0.02)
+    tol = 1e-11 if prec == glacx.FP64 else 1e-4
+    for i, r in enumerate(rows):
+        assert abs(w[i] - r[3]) < tol * max(1.0, abs(r[3])), i
+        assert np.abs(wt[i] - r[5]).max() < tol * max(1.0, np.abs(r[5]).max()), i
+        assert abs(q[i] - r[2]) < tol * 10
+
+
+def test_compressed_mode_refuses_non_su3_links(tmp_path):
+    """Two stored rows rebuild the third only for SU(3) links: GLAC's
+    generateRandom hot start (det = -1) must be refused, not mangled."""
+    n = (4, 4, 4, 4)
+    banner("compressed mode refuses det = -1")
+    U = O.new_field(n)
+    rng = O.GlacRng(3, 4)
+    O.glac_hot_start(U, n, rng, 0.24)
+    assert np.allclose(np.linalg.det(O.to_complex(U[0, :4])), -1.0)
+    path = str(tmp_path / "glac_hot.bin")
+    np.ascontiguousarray(U.reshape(4, -1, 18).transpose(1, 0, 2)).tofile(path)
+    with glacx.Lattice(n, glacx.FP32_COMPRESSED) as L:
+        with pytest.raises(glacx.GlacxError, match="SU\\(3\\)"):
+            L.upload(U)
+        with pytest.raises(glacx.GlacxError, match="SU\\(3\\)"):
+            L.load_bin(path)
+    with glacx.Lattice(n, glacx.FP32_LINKS) as L:     # the uncompressed mode takes them
+        L.upload(U)
+        assert np.abs(L.download() - U).max() < 1e-6
 
 
 def test_errors_are_reported():

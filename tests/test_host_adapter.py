@@ -34,10 +34,10 @@ def test_delta_action_through_adapter():
 
 
 @pytest.mark.gpu
-def test_flow_measure_metropolis_through_adapter():
+def test_flow_measure_metropolis_through_adapter(tmp_path):
     exe = build_check()
     print("\n[gpu case] host adapter flow/measure/metropolis 4x4x4x8 fp64", flush=True)
-    out = subprocess.run([exe, "gpu"], capture_output=True, text=True, timeout=120)
+    out = subprocess.run([exe, "gpu", str(tmp_path)], capture_output=True, text=True, timeout=120)
     print(out.stdout)
     assert out.returncode == 0, out.stdout + out.stderr
-    assert out.stdout.count("PASS") == 3
+    assert out.stdout.count("PASS") == 4

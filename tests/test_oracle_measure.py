@@ -62,3 +62,37 @@ def test_time_reflection_flips_charge():
     assert abs(a[0] - b[0]) < 1e-14 and abs(a[1] - b[1]) < 1e-11
     assert abs(a[2] + b[2]) < 1e-13
     assert abs(a[2]) > 1e-4  # the check is not vacuous
+
+
+# ---- GLAC's SuperSampler: the same observables plus the Weinberg operator
+
+
+def test_super_sampler_shares_the_clover_observables():
+    rng = np.random.default_rng(41)
+    n = (4, 4, 6, 4)
+    U = random_field(n, rng)
+    a = O.measure(U, n)
+    b = O.measure_super(U, n)
+    assert a[0] == b[0] and a[1] == b[1] and a[2] == b[2] and np.array_equal(a[3], b[4])
+    assert abs(b[5].sum() - b[3]) < 1e-12 * max(1.0, abs(b[3]))
+
+
+def test_weinberg_gauge_invariant_and_odd_under_time_reflection():
+    rng = np.random.default_rng(42)
+    n = (4, 4, 4, 6)
+    U = random_field(n, rng)
+    w = O.measure_super(U, n)[3]
+    g = random_su3(rng, (n[3], n[2], n[1], n[0]))
+    wg = O.measure_super(gauge_transform(U, n, g), n)[3]
+    wr = O.measure_super(reflect_t(U, n), n)[3]
+    assert abs(w) > 1e-3                      # not vacuous
+    assert abs(w - wg) < 1e-12 * abs(w)
+    assert abs(w + wr) < 1e-12 * abs(w)
+
+
+def test_weinberg_vanishes_on_unit_and_abelian_fields():
+    n = (4, 4, 4, 8)
+    assert O.measure_super(O.unit_field(n), n)[3] == 0.0
+    U, phi1, phi2 = abelian_field(n, 1, 1)
+    _, _, q, w, _, wt = O.measure_super(U, n)
+    assert abs(q) > 0.5 and abs(w) < 1e-12 and np.abs(wt).max() < 1e-12
