@@ -54,6 +54,18 @@ def metropolis_bytes_per_launch(V, fp32, rows=3):
     return 4.5 * V * link_bytes(fp32, rows)
 
 
+def profiled_traffic(n, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE measurement (profiles/traffic_<lattice>.json),
+    or None when that lattice was not profiled."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % "x".join(map(str, n)))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        entry = json.load(f).get(kernel)
+    return entry["traffic_bytes_per_launch"] if entry else None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -197,7 +209,7 @@ def main():
         "config": {"workload": "RK3 Wilson flow step + clover measurement", "lattice": list(n),
                    "beta": beta, "flow_eps": eps_flow, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(n, f"flow_stage_kernel<float, {rows}>"),
                      "kernel": f"flow_stage_kernel<float, {rows}>",
                      "alg_bytes_per_launch": stage_bytes, "mean_launch_ms": stage_ms},
         "metropolis": {"sweeps_per_s": sweeps_per_s, "acceptance": float(acc.mean()),

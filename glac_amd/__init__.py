@@ -62,6 +62,7 @@ def library():
         "glacx_save_bin": ([_vp, C.c_int, C.c_char_p], C.c_int),
         "glacx_load_chroma": ([_vp, C.c_int, C.c_char_p], C.c_int),
         "glacx_measure_super": ([_vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int),
+        "glacx_measure_density": ([_vp, C.c_int, _dp, _dp, _dp, _dp, _dp], C.c_int),
         "glacx_flow_super": ([_vp, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int),
         "glacx_cold_start": ([_vp, C.c_int], C.c_int),
         "glacx_hot_start": ([_vp, C.c_int, C.c_uint64], C.c_int),
@@ -208,6 +209,15 @@ class Lattice:
         self._check(self._L.glacx_measure_super(self._h, field, C.byref(p), C.byref(e), C.byref(q), C.byref(w),
                                                 _ptr(qt), _ptr(wt)), "measure_super")
         return p.value, e.value, q.value, w.value, qt, wt
+
+    def measure_density(self, field=MAIN):
+        """Per-site (charge, action) densities in GLAC's site order, plus
+        (plaq, E_raw, Q): GLAC's LatticeActionChargeDensity."""
+        qd, ed = np.empty(self.V), np.empty(self.V)
+        p, e, q = C.c_double(), C.c_double(), C.c_double()
+        self._check(self._L.glacx_measure_density(self._h, field, _ptr(qd), _ptr(ed), C.byref(p), C.byref(e),
+                                                  C.byref(q)), "measure_density")
+        return qd, ed, p.value, e.value, q.value
 
     def plaquette(self, field=MAIN) -> float:
         p = C.c_double()

@@ -148,16 +148,18 @@ int timed_end(glacx_ctx *c, int kernel, hipEvent_t start)
 }
 
 // ---- kernel sequences ----------------------------------------------------
-template <class T, int R> int run_measure(glacx_ctx *c, const void *f, double *out, bool super)
+// mode: 0 plain clover observables, 1 with the Weinberg operator, 2 with
+// per-site densities written to `density` (2V doubles).
+template <class T, int R> int run_measure(glacx_ctx *c, const void *f, double *out, int mode, double *density = nullptr)
 {
     int rc = ensure(c, &c->partials, &c->partials_len, (size_t)measure_partials_count(c->geom));
     if (rc) return rc;
     hipEvent_t e0 = nullptr;
     if ((rc = timed_begin(c, &e0))) return rc;
-    launch_measure<T, R>(f, c->geom, c->partials, super, c->stream);
+    launch_measure<T, R>(f, c->geom, c->partials, mode, density, c->stream);
     CHECK_LAUNCH(c);
     if ((rc = timed_end(c, GLACX_K_MEASURE, e0))) return rc;
-    launch_measure_finish(c->partials, c->geom, out, super, c->stream);
+    launch_measure_finish(c->partials, c->geom, out, mode == 1, c->stream);
     CHECK_LAUNCH(c);
     return GLACX_OK;
 }
@@ -555,7 +557,7 @@ int measure_impl(glacx_ctx *c, int field, bool super, double *plaq, double *e_ra
     const size_t row = super ? (size_t)4 + 2 * nt : (size_t)3 + nt;
     int rc = ensure(c, &c->results, &c->results_len, row);
     if (rc) return rc;
-    if ((rc = DISPATCH(c, run_measure, c, c->field[field], c->results, super))) return rc;
+    if ((rc = DISPATCH(c, run_measure, c, c->field[field], c->results, super ? 1 : 0))) return rc;
     std::vector<double> h(row);
     HIP_TRY(c, hipMemcpyAsync(h.data(), c->results, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -584,6 +586,28 @@ int glacx_measure_super(glacx_ctx *c, int field, double *plaq, double *e_raw, do
     return measure_impl(c, field, true, plaq, e_raw, q, w, q_t, w_t);
 }
 
+int glacx_measure_density(glacx_ctx *c, int field, double *q_density, double *e_density, double *plaq,
+                          double *e_raw, double *q)
+{
+    CTX_CHECK(c);
+    if (!valid_field(field)) return fail(c, GLACX_E_ARG, "measure_density: bad field");
+    int rc = ensure_stage(c);
+    if (rc) return rc;
+    if ((rc = ensure(c, &c->results, &c->results_len, (size_t)3 + c->dims[3]))) return rc;
+    if ((rc = DISPATCH(c, run_measure, c, c->field[field], c->results, 2, c->stage))) return rc;
+    const size_t v = (size_t)c->geom.v;
+    double head[3];
+    HIP_TRY(c, hipMemcpyAsync(head, c->results, sizeof head, hipMemcpyDeviceToHost, c->stream));
+    if (q_density) HIP_TRY(c, hipMemcpyAsync(q_density, c->stage, v * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (e_density)
+        HIP_TRY(c, hipMemcpyAsync(e_density, c->stage + v, v * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (plaq) *plaq = head[0];
+    if (e_raw) *e_raw = head[1];
+    if (q) *q = head[2];
+    return GLACX_OK;
+}
+
 int glacx_plaquette(glacx_ctx *c, int field, double *plaq)
 {
     CTX_CHECK(c);
@@ -607,13 +631,13 @@ int flow_impl(glacx_ctx *c, int field, int n_steps, double eps, bool super, doub
     int rc;
     if (meas && (rc = ensure(c, &c->results, &c->results_len, row * (size_t)(n_steps + 1)))) return rc;
     void *f = c->field[field];
-    if (meas && (rc = DISPATCH(c, run_measure, c, f, c->results, super))) return rc;
+    if (meas && (rc = DISPATCH(c, run_measure, c, f, c->results, super ? 1 : 0))) return rc;
     for (int i = 0; i < n_steps; i++) {
         if ((rc = DISPATCH(c, run_flow_step, c, f, eps))) return rc;
         // three stages leave the result in the work buffer: swap roles
         std::swap(c->field[field], c->work);
         f = c->field[field];
-        if (meas && (rc = DISPATCH(c, run_measure, c, f, c->results + row * (size_t)(i + 1), super))) return rc;
+        if (meas && (rc = DISPATCH(c, run_measure, c, f, c->results + row * (size_t)(i + 1), super ? 1 : 0))) return rc;
     }
     if (meas) {
         std::vector<double> h(row * (size_t)(n_steps + 1));

@@ -105,10 +105,21 @@ constexpr int kMeasAcc = 4;
 // forms its two terms with lambda_a = next(nu), lambda_b = next(lambda_a):
 // G_{nu lambda_a} is stored as such (slot 2 + lambda_b) and
 // G_{nu lambda_b} = G_{lambda_b nu}^dagger (slot 2 + lambda_a).
-template <class T, int R, bool SUPER>
+//
+// DENSITY also writes the per-site charge and action densities of GLAC's
+// LatticeActionChargeDensity (GLAC/observables/latticeactionchargedensity.cpp:
+// 169-306): q(x) = -sum_nu Re tr(G_{0 nu} G_{rho sigma}) / (256 pi^2) and
+// e(x) = sum_nu [Re tr(G_{0 nu}^2) + Re tr(G_{rho sigma}^2)], at GLAC's
+// lexicographic site index; the three wavefronts' terms of a site are
+// added through LDS in the order nu = 1, 2, 3.
+enum MeasMode { MEAS_PLAIN = 0, MEAS_SUPER = 1, MEAS_DENSITY = 2 };
+
+template <class T, int R, int MODE>
 __global__ __launch_bounds__(kMeasSites *kMeasWaves) void measure_kernel(const typename vec2<T>::type *__restrict__ f,
-                                                                         Geom g, int cps, double *__restrict__ partials)
+                                                                         Geom g, int cps, double *__restrict__ partials,
+                                                                         double *__restrict__ density)
 {
+    constexpr bool SUPER = MODE == MEAS_SUPER, DENSITY = MODE == MEAS_DENSITY;
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int rest = fdiv(b, g.d_cpr), jc = b - rest * g.cpr;
     const int lt = rest & ((1 << g.lbt) - 1), blk = rest >> g.lbt;
@@ -120,8 +131,10 @@ __global__ __launch_bounds__(kMeasSites *kMeasWaves) void measure_kernel(const t
     const bool valid = k < g.run;
     double acc[kMeasAcc] = {0.0, 0.0, 0.0, 0.0};
     M3<T> c2;
+    int lex = 0;
     if (valid) {
         const Coord x = decode(g, run_index(g, blk, lane >> 5, lt, k));
+        lex = lex_of(g, x);
         const int rho = nu % 3 + 1, sigma = rho % 3 + 1;
         T p1, p2;
         const M3<T> c1 = traceless_antiherm2(clover<T, R>(f, g, x, 0, nu, p1));
@@ -141,6 +154,17 @@ __global__ __launch_bounds__(kMeasSites *kMeasWaves) void measure_kernel(const t
             const double ta = retrace_mul(mul(lds_get<T>(la - 1, lane), lds_get<T>(2 + lb, lane)), c2);
             const double tb = retrace_mul(mul_ad(lds_get<T>(lb - 1, lane), lds_get<T>(2 + la, lane)), c2);
             acc[3] = -(ta + tb);
+        }
+    }
+    if constexpr (DENSITY) {
+        __shared__ double dens[2][kMeasWaves][kMeasSites];
+        dens[0][nu - 1][lane] = acc[2];
+        dens[1][nu - 1][lane] = acc[1];
+        __syncthreads();
+        if (nu == 1 && valid) {
+            const double qnorm = 1.0 / (256.0 * M_PI * M_PI);
+            density[lex] = ((dens[0][0][lane] + dens[0][1][lane]) + dens[0][2][lane]) * qnorm;
+            density[(size_t)g.v + lex] = (dens[1][0][lane] + dens[1][1][lane]) + dens[1][2][lane];
         }
     }
     block_sum<kMeasAcc, kMeasWaves>(acc);
@@ -245,14 +269,18 @@ static inline int chunks_per_slab(const Geom &g) { return g.nzb * g.cpr; }
 int measure_partials_count(const Geom &g) { return kMeasAcc * g.n[3] * chunks_per_slab(g) + kMeasAcc * g.n[3]; }
 int plaquette_partials_count(const Geom &g) { return (g.v + kBlock - 1) / kBlock; }
 
-template <class T, int R> void launch_measure(const void *f, const Geom &g, double *partials, bool super, hipStream_t s)
+template <class T, int R>
+void launch_measure(const void *f, const Geom &g, double *partials, int mode, double *density, hipStream_t s)
 {
     const int cps = chunks_per_slab(g);
     const auto *fp = (const typename vec2<T>::type *)f;
-    if (super)
-        measure_kernel<T, R, true><<<g.n[3] * cps, kMeasSites * kMeasWaves, 0, s>>>(fp, g, cps, partials);
+    const unsigned blocks = (unsigned)(g.n[3] * cps), threads = kMeasSites * kMeasWaves;
+    if (mode == MEAS_SUPER)
+        measure_kernel<T, R, MEAS_SUPER><<<blocks, threads, 0, s>>>(fp, g, cps, partials, density);
+    else if (mode == MEAS_DENSITY)
+        measure_kernel<T, R, MEAS_DENSITY><<<blocks, threads, 0, s>>>(fp, g, cps, partials, density);
     else
-        measure_kernel<T, R, false><<<g.n[3] * cps, kMeasSites * kMeasWaves, 0, s>>>(fp, g, cps, partials);
+        measure_kernel<T, R, MEAS_PLAIN><<<blocks, threads, 0, s>>>(fp, g, cps, partials, density);
 }
 
 void launch_measure_finish(double *partials, const Geom &g, double *out, bool super, hipStream_t s)
@@ -273,9 +301,9 @@ void launch_plaquette_finish(const double *partials, int nparts, const Geom &g, 
     plaquette_finish_kernel<<<1, kBlock, 0, s>>>(partials, nparts, g, out);
 }
 
-template void launch_measure<float, 3>(const void *, const Geom &, double *, bool, hipStream_t);
-template void launch_measure<float, 2>(const void *, const Geom &, double *, bool, hipStream_t);
-template void launch_measure<double, 3>(const void *, const Geom &, double *, bool, hipStream_t);
+template void launch_measure<float, 3>(const void *, const Geom &, double *, int, double *, hipStream_t);
+template void launch_measure<float, 2>(const void *, const Geom &, double *, int, double *, hipStream_t);
+template void launch_measure<double, 3>(const void *, const Geom &, double *, int, double *, hipStream_t);
 template void launch_plaquette<float, 3>(const void *, const Geom &, double *, hipStream_t);
 template void launch_plaquette<float, 2>(const void *, const Geom &, double *, hipStream_t);
 template void launch_plaquette<double, 3>(const void *, const Geom &, double *, hipStream_t);

@@ -63,9 +63,12 @@ void launch_action_derivative(const void *f, double *stage, const Geom &g, int m
 
 // Clover measurement: per-block partial sums, then a one-block fixed-order
 // reduction into out = {plaq, E_raw, Q, Q(t)...}, or with `super` (GLAC's
-// SuperSampler) {plaq, E_raw, Q, W, Q(t)..., W(t)...}.
+// SuperSampler) {plaq, E_raw, Q, W, Q(t)..., W(t)...}.  mode: 0 plain,
+// 1 SuperSampler (Weinberg), 2 plain plus per-site densities into
+// density[0..V) (charge) and density[V..2V) (action), GLAC site order.
 int measure_partials_count(const Geom &g);
-template <class T, int R> void launch_measure(const void *f, const Geom &g, double *partials, bool super, hipStream_t s);
+template <class T, int R>
+void launch_measure(const void *f, const Geom &g, double *partials, int mode, double *density, hipStream_t s);
 void launch_measure_finish(double *partials, const Geom &g, double *out, bool super, hipStream_t s);
 
 int plaquette_partials_count(const Geom &g);

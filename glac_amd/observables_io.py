@@ -15,6 +15,10 @@ GPU run can replace a GLAC run without touching that analysis:
       :29-66, with ObservableStorer::runStatistics,
       GLAC/observables/tools/observablestorer.cpp:77-98).
 
+  <out>/<batch>/scalar_fields/<observable>/<config name>.bin
+      per-site densities (topc, energy) as raw doubles in GLAC's site order
+      (IO::FieldIO::writeDoublesFieldToFile, GLAC/io/fieldio.cpp:137-186).
+
 Number formatting follows the C++ streams GLAC writes with: header values
 in the default format (6 significant digits, like printf's %g), rows in
 std::fixed with 15 decimals (printf's %.15f).  Observable names are GLAC's
@@ -115,6 +119,19 @@ def write_observables(out: str, batch: str, name: str, values: Sequence[float], 
         f.write(f"Average{name} {_f15(avg)}\nVariance{name} {_f15(var)}\nstd{name} {_f15(std)}\n")
         for v in values:
             f.write(_f15(float(v)) + "\n")
+    return path
+
+
+def write_scalar_field(out: str, batch: str, observable: str, config: int, values, beta: float, n_spatial: int,
+                       n_temporal: int, num_proc: int = 1, config_start: int = 0) -> str:
+    """A per-site density (V doubles, GLAC's site order, native-endian, no
+    header) as IO::FieldIO::writeDoublesFieldToFile writes it
+    (GLAC/io/fieldio.cpp:137-186) under <out>/<batch>/scalar_fields/<observable>/."""
+    import numpy as np
+    path = os.path.join(out, batch, "scalar_fields", observable,
+                        config_filename(batch, beta, n_spatial, n_temporal, config, num_proc, config_start))
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.ascontiguousarray(values, dtype=np.float64).tofile(path)
     return path
 
 

@@ -123,6 +123,17 @@ int glacx_measure(glacx_ctx *ctx, int field, double *plaq, double *e_raw, double
 int glacx_measure_super(glacx_ctx *ctx, int field, double *plaq, double *e_raw, double *q, double *w, double *q_t,
                         double *w_t);
 
+/* Per-site charge and action densities of GLAC's LatticeActionChargeDensity
+ * (GLAC/observables/latticeactionchargedensity.cpp:169-306), each V doubles in
+ * GLAC's site order x + Nx(y + Ny(z + Nz t)): q_density(x) = -sum_nu
+ * Re tr(G_{0 nu} G_{rho sigma}) / (256 pi^2) (sums to Q) and e_density(x) =
+ * sum_nu [Re tr(G_{0 nu}^2) + Re tr(G_{rho sigma}^2)] (sums to V E_raw), the
+ * fields GLAC writes with IO::FieldIO::writeDoublesFieldToFile
+ * (GLAC/io/fieldio.cpp:137-186).  Also returns plaq, e_raw, q.  Any pointer
+ * may be NULL. */
+int glacx_measure_density(glacx_ctx *ctx, int field, double *q_density, double *e_density, double *plaq,
+                          double *e_raw, double *q);
+
 /* Plaquette only (Plaquette::calculate, GLAC/observables/plaquette.cpp:26-40). */
 int glacx_plaquette(glacx_ctx *ctx, int field, double *plaq);
 

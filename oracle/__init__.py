@@ -66,6 +66,7 @@ def lib():
         L.or_flow_step.argtypes = [_dp, _i4, C.c_double]
         L.or_measure.argtypes = [_dp, _i4, _dp, _dp]
         L.or_measure_super.argtypes = [_dp, _i4, _dp, _dp, _dp]
+        L.or_measure_density.argtypes = [_dp, _i4, _dp, _dp]
         L.or_glac_rng_new.argtypes = [C.c_uint64, C.c_uint64]
         L.or_glac_rng_new.restype = C.c_void_p
         L.or_glac_rng_free.argtypes = [C.c_void_p]
@@ -246,6 +247,14 @@ def measure(U, n):
     qt = np.empty(int(n[3]))
     lib().or_measure(np.ascontiguousarray(U), dims4(n), out, qt)
     return float(out[0]), float(out[1]), float(out[2]), qt
+
+
+def measure_density(U, n):
+    """Per-site (charge, action) densities, GLAC's site order."""
+    V = volume(n)
+    qd, ed = np.empty(V), np.empty(V)
+    lib().or_measure_density(np.ascontiguousarray(U), dims4(n), qd, ed)
+    return qd, ed
 
 
 def measure_super(U, n):

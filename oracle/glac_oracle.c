@@ -51,6 +51,10 @@
  *      Pinned: unit field (plaq = 1, E = Q = 0); constant abelian fields
  *      with closed-form plaquette and energy and integer charge Q; gauge
  *      invariance; sum_t Q(t) = Q; parity flips Q.
+ *  or_measure_density   GLAC/observables/latticeactionchargedensity.cpp:
+ *      169-306.  Pinned: sums equal or_measure's Q and V E_raw; every site's
+ *      values are gauge invariant; a lattice translation of the field
+ *      translates the densities; time reflection flips the charge density.
  *  or_measure_super   GLAC/observables/supersampler.cpp:298-496 (adds the
  *      Weinberg operator W and W(t) to or_measure's observables).  Pinned:
  *      plaq, E, Q, Q(t) equal or_measure's; W = 0 on commuting (abelian)
@@ -666,6 +670,34 @@ void or_measure(const double *U, const int n[4], double out[3], double *qt)
     out[0] = plaq / (18.0 * (double)V);
     out[1] = energy / (double)V;
     out[2] = q;
+}
+
+/* Per-site densities of GLAC's LatticeActionChargeDensity
+ * (GLAC/observables/latticeactionchargedensity.cpp:169-306), lexicographic
+ * site order: qd = -sum_nu Re tr(G_{0 nu} G_{rho sigma}) / (256 pi^2),
+ * ed = sum_nu [Re tr(G_{0 nu}^2) + Re tr(G_{rho sigma}^2)]. */
+void or_measure_density(const double *U, const int n[4], double *qd, double *ed)
+{
+    long V = volume(n);
+    const double qnorm = 1.0 / (256.0 * M_PI * M_PI);
+    for (long s = 0; s < V; s++) {
+        int c[4];
+        decode(n, s, c);
+        double qs = 0.0, es = 0.0;
+        for (int nu = 1; nu < 4; nu++) {
+            int rho = nu % 3 + 1, sigma = rho % 3 + 1;
+            double C1[18], C2[18];
+            clover(U, n, c, 0, nu, C1);
+            clover(U, n, c, rho, sigma, C2);
+            traceless_antiherm(C1);
+            traceless_antiherm(C2);
+            qs -= or_su3_retrace_mul(C1, C2);
+            es += or_su3_retrace_mul(C1, C1);
+            es += or_su3_retrace_mul(C2, C2);
+        }
+        qd[s] = qs * qnorm;
+        ed[s] = es;
+    }
 }
 
 /* GLAC's SuperSampler: the clover observables plus the Weinberg operator

@@ -60,6 +60,7 @@ void or_flow_step(double *U, const int n[4], double eps);
 
 /* ---- clover observables ------------------------------------------------- */
 void or_measure(const double *U, const int n[4], double out[3], double *qt);
+void or_measure_density(const double *U, const int n[4], double *qd, double *ed);
 void or_measure_super(const double *U, const int n[4], double out[4], double *qt, double *wt);
 
 /* ---- Metropolis: GLAC's sequential sweep with its two mt19937_64 streams -- */

@@ -72,3 +72,10 @@ def test_run_observables_and_statistics(tmp_path):
 
 def test_config_filename():
     assert io.config_filename("b", 6.0, 8, 16, 7) == "b_b6.000000_N8_NT16_np1_config00007.bin"
+
+
+def test_scalar_field_file(tmp_path):
+    vals = np.arange(4 * 4 * 4 * 8, dtype=float) / 3.0
+    path = io.write_scalar_field(str(tmp_path), "run", "topc", 2, vals, 6.0, 4, 8)
+    assert path.endswith(os.path.join("run", "scalar_fields", "topc", "run_b6.000000_N4_NT8_np1_config00002.bin"))
+    assert np.array_equal(np.fromfile(path, dtype=np.float64), vals)

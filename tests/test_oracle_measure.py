@@ -96,3 +96,26 @@ def test_weinberg_vanishes_on_unit_and_abelian_fields():
     U, phi1, phi2 = abelian_field(n, 1, 1)
     _, _, q, w, _, wt = O.measure_super(U, n)
     assert abs(q) > 0.5 and abs(w) < 1e-12 and np.abs(wt).max() < 1e-12
+
+
+# ---- GLAC's LatticeActionChargeDensity: per-site densities
+
+
+def test_densities_sum_to_the_totals_and_are_local_invariants():
+    rng = np.random.default_rng(43)
+    n = (4, 6, 4, 4)
+    V = int(np.prod(n))
+    U = random_field(n, rng)
+    qd, ed = O.measure_density(U, n)
+    _, e, q, _ = O.measure(U, n)
+    assert abs(qd.sum() - q) < 1e-12 and abs(ed.sum() / V - e) < 1e-11 * abs(e)
+    g = random_su3(rng, (n[3], n[2], n[1], n[0]))
+    qg, eg = O.measure_density(gauge_transform(U, n, g), n)
+    assert np.abs(qg - qd).max() < 1e-13 and np.abs(eg - ed).max() < 1e-10
+    # translating the field by one site in y translates the densities
+    from fieldtools import from_grid, to_grid
+    Ut = from_grid(np.roll(to_grid(U, n), 1, axis=3))          # grid axes (mu, t, z, y, x, 3, 3)
+    qt, et = O.measure_density(Ut, n)
+    grid = lambda a: a.reshape(n[3], n[2], n[1], n[0])
+    assert np.abs(grid(qt) - np.roll(grid(qd), 1, axis=2)).max() < 1e-14
+    assert np.abs(grid(et) - np.roll(grid(ed), 1, axis=2)).max() < 1e-11
