@@ -66,6 +66,58 @@ def profiled_traffic(n, kernel):
     return entry["traffic_bytes_per_launch"] if entry else None
 
 
+def independent_configs(G, n, beta, seconds):
+    """Effectively independent configurations per second,
+    configs/s / (2 tau_int(Q^2)), for the learned proposal with FP32 links
+    (compressed) and GLAC's fixed-eps RST proposal with FP64 links.
+
+    Each variant: Haar hot start, 1500 RST sweeps; the learned variant then
+    collects 20 sweeps of (staple, link) samples and trains its network
+    (3000 Adam steps, not counted in configs/s); then 300 more sweeps of its
+    own update and configurations spaced by NCor = 50 sweeps, each flowed
+    50 RK3 steps of 0.02 (t = 1) and measured, for about `seconds` of wall
+    time.  tau_int(Q^2) by Wolff's Gamma method, in units of the spacing."""
+    from glac_amd.system import EnsembleConfig, run_ensemble
+    from glac_amd.proposal import train_from_chain
+    out = {}
+    for name, prec, proposal in [("fp32c_learned", G.FP32_COMPRESSED, "learned"), ("fp64_plain", G.FP64, "rst")]:
+        L = G.Lattice(n, prec)
+        L.hot_start(31)
+        L.metropolis(1500, beta, 0.24, 10, seed=3)
+        off = 1500
+        train_s = 0.0
+        if proposal == "learned":
+            t0 = time.perf_counter()
+            train_from_chain(L, beta, seed=3, sweep_offset=off, collect_sweeps=20)
+            L.synchronize()
+            train_s = time.perf_counter() - t0
+            off += 20
+        cfg = EnsembleConfig(dims=n, beta=beta, NTherm=300, NCf=1, NCor=50, NFlows=50, flowEpsilon=0.02, seed=9,
+                             proposal=proposal)
+        pilot = run_ensemble(L, cfg, thermalise=True, sweep_offset=off)
+        off += cfg.NTherm + cfg.NCor
+        cfg.NCf = max(40, int(seconds / (pilot.update_seconds + pilot.flow_seconds)))
+        r = run_ensemble(L, cfg, thermalise=False, sweep_offset=off,
+                         progress=lambda i, q: log(f"[bench] {name} config {i}") if i % 100 == 0 else None)
+        L.close()
+        ti = r.tau_int_q2()
+        rate = r.configs_per_second / (2.0 * ti["tau_int"])
+        out[name] = {"indep_configs_per_s": rate, "configs_per_s": r.configs_per_second,
+                     "tau_int_q2": ti["tau_int"], "tau_int_q2_err": ti["dtau"], "configs": len(r.q),
+                     "q2_mean": float(np.mean(r.q ** 2)), "q2_err": ti["err"], "plaq_mean": float(r.plaq.mean()),
+                     "acceptance": r.acceptance, "train_seconds": train_s}
+        log(f"[bench] {name}: {r.configs_per_second:.2f} configs/s, tau_int(Q^2) {ti['tau_int']:.2f} "
+            f"+- {ti['dtau']:.2f}, {rate:.3f} indep. configs/s, <Q^2> {np.mean(r.q ** 2):.2f}")
+    a, b = out["fp32c_learned"], out["fp64_plain"]
+    ratio = a["indep_configs_per_s"] / b["indep_configs_per_s"]
+    rel = np.hypot(a["tau_int_q2_err"] / a["tau_int_q2"], b["tau_int_q2_err"] / b["tau_int_q2"])
+    out["ratio_learned_fp32_over_plain_fp64"] = ratio
+    out["ratio_err"] = ratio * rel
+    out["protocol"] = ("NCor 50 sweeps (10 hits, eps 0.24 fallback/RST), flow 50 x 0.02, Q at t = 1; "
+                       "tau_int(Q^2) by Wolff's Gamma method (S = 1.5), errors from its estimate")
+    return out
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -81,6 +133,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--precision", choices=["fp32", "fp32c"], default="fp32c",
                     help="FP32 mode of the timed flow and sweeps: fp32 stores 3 rows per link, fp32c two")
+    ap.add_argument("--indep-seconds", type=float, default=45.0,
+                    help="wall seconds of ensemble generation per variant for the independent-configs/s "
+                         "measurement (learned FP32 vs plain FP64); 0 skips it")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy kernel (for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -178,6 +233,11 @@ def main():
         }
         log(f"[bench] flow fp64: {fp64['flow_steps_per_s']:.1f} steps/s, stage kernel {ms_stage64:.4f} ms")
 
+    # ---------------------------------------------------------------- indep. configs/s
+    indep = None
+    if args.indep_seconds > 0:
+        indep = independent_configs(G, n, beta, args.indep_seconds)
+
     # ---------------------------------------------------------------- CPU baseline
     cpu = None
     if not args.no_cpu_baseline:
@@ -217,7 +277,7 @@ def main():
                        "hbm_frac": metro_gbs / HBM_PEAK_GBS},
         "measure_kernel_ms": meas_ms / max(meas_launches, 1),
         "fp64_plain": fp64,
-        "indep_configs_per_s": None,
+        "indep_configs_per_s": indep,
         "cpu_baseline": cpu,
         "final_obs": {"plaq": float(plaq[-1]), "E": float(-e_raw[-1] / 64), "Q": float(q[-1])},
     }

@@ -27,19 +27,77 @@
 
 namespace glacx {
 
+// Links a block shares through LDS: wavefront mu loads U_mu(x) (its own
+// link) and U_mu(x - mu) for the block's 64 sites; the other three
+// wavefronts take them from here for their staples instead of gathering them
+// again (U_nu(x) closes every upper staple, U_nu(x - nu) every lower one).
+// Per site that is 56 link gathers from memory instead of 76.
+template <class T> struct FlowLds {
+    T own[4][18][kFlowSites];
+    T back[4][18][kFlowSites];
+};
+
+template <class T> __device__ __forceinline__ void lds_store(T (&s)[18][kFlowSites], int lane, const M3<T> &m)
+{
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+        s[2 * e][lane] = m.e[e / 3][e % 3].r;
+        s[2 * e + 1][lane] = m.e[e / 3][e % 3].i;
+    }
+}
+template <class T> __device__ __forceinline__ M3<T> lds_load(const T (&s)[18][kFlowSites], int lane)
+{
+    M3<T> m;
+#pragma unroll
+    for (int e = 0; e < 9; e++) m.e[e / 3][e % 3] = {s[2 * e][lane], s[2 * e + 1][lane]};
+    return m;
+}
+
 template <class T, int R, int STAGE>
 __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<T>::type *__restrict__ win,
                                                             typename vec2<T>::type *__restrict__ wout,
                                                             T *__restrict__ acc, Geom g, T cz, T ca)
 {
+    __shared__ FlowLds<T> sh;
     const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // one direction per wavefront
     const int lane = threadIdx.x & 63;
-    const int h = xcd_block(blockIdx.x, gridDim.x) * (kFlowSites / 2) + (lane & 31);
-    if (h >= g.vh) return;
+    const int h_raw = xcd_block(blockIdx.x, gridDim.x) * (kFlowSites / 2) + (lane & 31);
+    const bool valid = h_raw < g.vh;
+    const int h = valid ? h_raw : g.vh - 1;  // idle lanes of a partial block mirror a real site, store nothing
     const int idx = parity_index(g, lane >> 5, h);
     const Coord x = decode(g, idx);
-    const M3<T> a = staple<T, R>(win, g, x, mu);
-    const M3<T> u = load_link<T, R>(win, g, mu, idx);
+    const DirView<T> vmu = dir_view<T>(win, g, mu);
+    lds_store(sh.own[mu], lane, load_link<T, R>(vmu, idx));
+    lds_store(sh.back[mu], lane, load_link<T, R>(vmu, index_of(g, step(g, x, mu, -1))));
+    __syncthreads();
+    // staple A_mu(x) = sum_{nu != mu} [ U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
+    //                                 + U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu) ]
+    // (GLAC/actions/wilsongaugeaction.cpp:29-53)
+    M3<T> a = zero3<T>();
+    {
+        const Coord xpm = step(g, x, mu, +1);
+        const int ipm = index_of(g, xpm);
+#pragma unroll 1
+        for (int d = 1; d < 4; d++) {
+            const int nu = (mu + d) & 3;
+            const DirView<T> vnu = dir_view<T>(win, g, nu);
+            const int ipn = index_of(g, step(g, x, nu, +1));
+            const int imn = index_of(g, step(g, x, nu, -1));
+            const int ipmmn = index_of(g, step(g, xpm, nu, -1));
+            {
+                const M3<T> u1 = load_link<T, R>(vnu, ipm);
+                const M3<T> u2 = load_link<T, R>(vmu, ipn);
+                mm<false, true, true>(a, mul_ad(u1, u2), lds_load(sh.own[nu], lane));
+            }
+            {
+                const M3<T> u1 = load_link<T, R>(vnu, ipmmn);
+                const M3<T> u2 = load_link<T, R>(vmu, imn);
+                mm<false, false, true>(a, mul_dd(u1, u2), lds_load(sh.back[nu], lane));
+            }
+        }
+    }
+    if (!valid) return;
+    const M3<T> u = lds_load(sh.own[mu], lane);
     const su3alg<T> z = force_from_omega(mul(u, a));
     T *ap = acc + (size_t)mu * 8 * g.v + idx;
     su3alg<T> y;
