@@ -2,17 +2,22 @@
 """Benchmark of the GLAC hot path on one MI355X.
 
 Workload: a 32^4 periodic lattice at beta = 6.1 (the lattice of the headline
-metric).  A *step* is one RK3 Wilson-flow step (eps = 0.01) with FP32 links,
-FP64 exponential coefficients and FP64 reductions, followed by the clover
+metric).  A *step* is one RK3 Wilson-flow step (eps = 0.01) with FP32 links
+(two stored rows per link by default, ``--precision fp32`` for three), FP64
+exponential coefficients and FP64 reductions, followed by the clover
 measurement (plaquette, E, Q, Q(t)) -- one iteration of GLAC's
 System::flowConfiguration loop (GLAC/system.cpp:494-508).  ``value`` is flow
 steps per second with the field resident in HBM.
 
-Also reported, each from its own timed region on the same lattice:
+Also reported, each from its own timed region:
  - Metropolis sweeps/s (FP32 links, 10 hits, eps = 0.24, reunitarised every
    sweep) and FP64 flow steps/s;
  - ``roofline``: the flow-stage kernel's algorithmic bytes per launch over its
-   mean launch time (HIP events on the library's stream), against 8 TB/s;
+   mean launch time (HIP events on the library's stream), against 8 TB/s, with
+   the HBM traffic per launch measured by rocprofv3 (profiles/);
+ - ``indep_configs_per_s``: effectively independent configurations per second,
+   configs/s / (2 tau_int(Q^2)), for the learned proposal with FP32 links and
+   for GLAC's plain RST proposal with FP64 links (``--indep-seconds`` each);
  - ``cpu_baseline``: the FP64 oracle (one host thread) doing one flow step plus
    one measurement on the same 32^4 lattice.
 
@@ -136,6 +141,8 @@ def main():
     ap.add_argument("--indep-seconds", type=float, default=45.0,
                     help="wall seconds of ensemble generation per variant for the independent-configs/s "
                          "measurement (learned FP32 vs plain FP64); 0 skips it")
+    ap.add_argument("--only-indep", action="store_true",
+                    help="run only the independent-configs/s measurement and print its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy kernel (for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -163,6 +170,10 @@ def main():
     n = tuple(int(v) for v in args.dims.split(","))
     V = int(np.prod(n))
     beta, eps_flow = 6.1, 0.01
+
+    if args.only_indep:
+        print(json.dumps(independent_configs(G, n, beta, args.indep_seconds)), flush=True)
+        return
 
     # ---------------------------------------------------------------- FP32 flow
     rows = 2 if args.precision == "fp32c" else 3
