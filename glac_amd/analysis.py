@@ -66,3 +66,20 @@ def tau_int(a: np.ndarray, S: float = 1.5) -> dict:
 def effective_configs_per_second(configs_per_second: float, tau: float) -> float:
     """Independent configurations per second, (configs/s) / (2 tau_int)."""
     return configs_per_second / (2.0 * tau)
+
+
+def flow_scale(t: np.ndarray, e: np.ndarray, target: float = 0.3) -> float:
+    """The gradient-flow scale t0: the flow time at which t^2 E(t) first
+    reaches `target` (0.3, Luscher, JHEP 1008:071), by linear interpolation
+    of t^2 E between the bracketing flow times.  `e` is the physical energy
+    density E = -E_raw / 64 at flow times `t`.  Returns nan when t^2 E stays
+    below the target."""
+    t = np.asarray(t, dtype=np.float64)
+    f = t * t * np.asarray(e, dtype=np.float64)
+    above = np.nonzero(f >= target)[0]
+    if above.size == 0:
+        return float("nan")
+    i = int(above[0])
+    if i == 0:
+        return float(t[0])
+    return float(t[i - 1] + (target - f[i - 1]) * (t[i] - t[i - 1]) / (f[i] - f[i - 1]))

@@ -41,3 +41,15 @@ def test_white_noise_has_tau_one_half():
 
 def test_effective_configs():
     assert effective_configs_per_second(10.0, 2.5) == 2.0
+
+
+def test_flow_scale_interpolates_t2e():
+    from glac_amd.analysis import flow_scale
+    t = np.linspace(0.0, 5.0, 501)
+    # t^2 E = 0.1 t exactly: crosses 0.3 at t = 3
+    e = np.where(t > 0, 0.1 / np.where(t > 0, t, 1.0), 0.0)
+    assert abs(flow_scale(t, e) - 3.0) < 1e-12
+    # quadratic t^2 E = 0.05 t^2: crosses 0.3 at sqrt(6); linear interpolation error < step^2
+    e2 = np.full_like(t, 0.05)
+    assert abs(flow_scale(t, e2) - np.sqrt(6.0)) < 1e-4
+    assert np.isnan(flow_scale(t, np.zeros_like(t)))
