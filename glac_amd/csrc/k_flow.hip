@@ -86,15 +86,30 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
             const int ipn = index_of(g, step(g, x, nu, +1));
             const int imn = index_of(g, step(g, x, nu, -1));
             const int ipmmn = index_of(g, step(g, xpm, nu, -1));
-            {
-                const M3<T> u1 = load_link<T, R>(vnu, ipm);
-                const M3<T> u2 = load_link<T, R>(vmu, ipn);
-                mm<false, true, true>(a, mul_ad(u1, u2), lds_load(sh.own[nu], lane));
-            }
-            {
-                const M3<T> u1 = load_link<T, R>(vnu, ipmmn);
-                const M3<T> u2 = load_link<T, R>(vmu, imn);
-                mm<false, false, true>(a, mul_dd(u1, u2), lds_load(sh.back[nu], lane));
+            if constexpr (R == 2) {
+                // Two stored rows: form each SU(3) product's first two rows
+                // from operands of which only those are needed, and rebuild
+                // the product's third row once (36 multiply-adds fewer per
+                // product than rebuilding both operands).
+                M3<T> p;  // U_nu(x+mu) U_mu(x+nu)^dag
+                mm<false, true, false, 2>(p, load_rows01<T>(vnu, ipm), load_link<T, R>(vmu, ipn));
+                rebuild_third_row(p);
+                mm<false, true, true>(a, p, lds_load(sh.own[nu], lane));
+                M3<T> q;  // U_mu(x-nu) U_nu(x+mu-nu), whose dagger the lower staple uses
+                mm<false, false, false, 2>(q, load_rows01<T>(vmu, imn), load_link<T, R>(vnu, ipmmn));
+                rebuild_third_row(q);
+                mm<true, false, true>(a, q, lds_load(sh.back[nu], lane));
+            } else {
+                {
+                    const M3<T> u1 = load_link<T, R>(vnu, ipm);
+                    const M3<T> u2 = load_link<T, R>(vmu, ipn);
+                    mm<false, true, true>(a, mul_ad(u1, u2), lds_load(sh.own[nu], lane));
+                }
+                {
+                    const M3<T> u1 = load_link<T, R>(vnu, ipmmn);
+                    const M3<T> u2 = load_link<T, R>(vmu, imn);
+                    mm<false, false, true>(a, mul_dd(u1, u2), lds_load(sh.back[nu], lane));
+                }
             }
         }
     }
@@ -124,14 +139,15 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
             for (int k = 0; k < 6; k++) ap[(size_t)(k + 2) * g.v] = y.p[k];
         }
     }
-    M3<T> w = mul(exp_su3(y), u);
+    M3<T> w = zero3<T>();
+    mm<false, false, false, R>(w, exp_su3(y), u);  // two rows suffice when two are stored
     if constexpr (sizeof(T) == 4) {
         // FP32 links: project back onto SU(3) in FP64 every stage.  Without
         // it the rounding of exp(Y) W drifts the links off the group
         // (|U U^dag - 1| ~ 1e-4 after 1000 steps at 24^3 x 48), and that drift,
         // not the FP32 measurement, dominates the error of E(t) at large t.
         M3<double> d = convert<double>(w);
-        reunitarize(d);
+        reunitarize<R>(d);
         w = convert<T>(d);
     }
     store_link<T, R>(wout, g, mu, idx, w);

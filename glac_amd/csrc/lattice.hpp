@@ -287,6 +287,21 @@ template <class T, int R> __device__ __forceinline__ M3<T> load_link(const DirVi
     return m;
 }
 
+// The first two rows only (row 2 left unset), for products that need no more
+// of this operand.
+template <class T> __device__ __forceinline__ M3<T> load_rows01(const DirView<T> &d, int idx)
+{
+    using V2 = typename vec2<T>::type;
+    const unsigned off = (unsigned)idx * (unsigned)sizeof(V2);
+    M3<T> m;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const V2 z = buf_load(d, off, k);
+        m.e[k / 3][k % 3] = {z.x, z.y};
+    }
+    return m;
+}
+
 template <class T, int R> __device__ __forceinline__ void store_link(const DirView<T> &d, int idx, const M3<T> &m)
 {
     using V2 = typename vec2<T>::type;

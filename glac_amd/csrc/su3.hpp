@@ -64,13 +64,16 @@ template <class T> __device__ __forceinline__ M3<T> unit3()
 // Element (i, k) of the left operand is a[i][k], or conj(a[k][i]) when DA is
 // set; element (k, j) of the right operand is b[k][j], or conj(b[j][k]) when
 // DB is set.  ACC adds the product to c instead of overwriting it.
-template <bool DA, bool DB, bool ACC, class T>
+// NR < 3 computes only the first NR rows of the product (the rest of c is
+// left untouched): for a product of SU(3) matrices the third row follows from
+// the first two (rebuild_third_row in lattice.hpp).
+template <bool DA, bool DB, bool ACC, int NR = 3, class T>
 __device__ __forceinline__ void mm(M3<T> &c, const M3<T> &a, const M3<T> &b)
 {
     constexpr T sa = DA ? T(-1) : T(1);  // sign of the imaginary part of the left element
     constexpr T sb = DB ? T(-1) : T(1);
 #pragma unroll
-    for (int i = 0; i < 3; i++)
+    for (int i = 0; i < NR; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             T re, im;
@@ -279,7 +282,7 @@ template <class T> __device__ __forceinline__ M3<T> expand(const su3alg<T> &y)
 }
 
 // Row Gram-Schmidt in FP64; row 2 = conj(row0 x row1) so det = +1.
-__device__ __forceinline__ void reunitarize(M3<double> &m)
+template <int ROWS = 3> __device__ __forceinline__ void reunitarize(M3<double> &m)
 {
     double n0 = 0.0;
 #pragma unroll
@@ -311,12 +314,14 @@ __device__ __forceinline__ void reunitarize(M3<double> &m)
         m.e[1][j].r *= n1;
         m.e[1][j].i *= n1;
     }
+    if constexpr (ROWS == 3) {  // ROWS == 2: the caller stores two rows only
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-        const int k = (j + 1) % 3, l = (j + 2) % 3;
-        cpx<double> x = cmul(m.e[0][k], m.e[1][l]);
-        cpx<double> y = cmul(m.e[0][l], m.e[1][k]);
-        m.e[2][j] = {x.r - y.r, -(x.i - y.i)};
+        for (int j = 0; j < 3; j++) {
+            const int k = (j + 1) % 3, l = (j + 2) % 3;
+            cpx<double> x = cmul(m.e[0][k], m.e[1][l]);
+            cpx<double> y = cmul(m.e[0][l], m.e[1][k]);
+            m.e[2][j] = {x.r - y.r, -(x.i - y.i)};
+        }
     }
 }
 
