@@ -104,6 +104,77 @@ template <int N> __device__ __forceinline__ ExpCoeffs exp_coefficients_series(do
     return r;
 }
 
+// FP32 variant of the series for the learned proposal's step matrix with FP32
+// links: 16 terms bound the truncation by ||H||^16 / 16! < 3e-9 for
+// 2 c1 = ||H||_F^2 <= 4, below FP32 rounding.
+template <int N> __device__ __forceinline__ void exp_coefficients_series_f32(float c0, float c1, float f[3][2])
+{
+    float a = 1.f, b = 0.f, d = 0.f;
+    f[0][0] = 1.f;
+    f[0][1] = f[1][0] = f[1][1] = f[2][0] = f[2][1] = 0.f;
+#pragma unroll
+    for (int n = 1; n <= N; n++) {
+        const float an = c0 * d, bn = fmaf(c1, d, a);
+        d = b;
+        a = an;
+        b = bn;
+        const float s = (float)((n & 2) ? -kInvFact[n] : kInvFact[n]);
+        const int part = n & 1;
+        f[0][part] = fmaf(s, a, f[0][part]);
+        f[1][part] = fmaf(s, b, f[1][part]);
+        f[2][part] = fmaf(s, d, f[2][part]);
+    }
+}
+
+// X = f0 1 + f1 (-iY) + f2 (-Y^2) from the coefficients.
+template <class T> __device__ __forceinline__ M3<T> exp_assemble(const M3<T> &Y, T f0r, T f0i, T f1r, T f1i, T f2r, T f2i)
+{
+    const M3<T> Y2 = square_antiherm(Y);
+    M3<T> x;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            // f1 * (-i Y) = (f1r + i f1i)(Y.i - i Y.r) ;  -f2 Y2
+            const T hr = Y.e[i][j].i, hi = -Y.e[i][j].r;
+            x.e[i][j].r = fma(f1r, hr, fma(-f1i, hi, fma(-f2r, Y2.e[i][j].r, f2i * Y2.e[i][j].i)));
+            x.e[i][j].i = fma(f1r, hi, fma(f1i, hr, fma(-f2r, Y2.e[i][j].i, -f2i * Y2.e[i][j].r)));
+        }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        x.e[i][i].r += f0r;
+        x.e[i][i].i += f0i;
+    }
+    return x;
+}
+
+template <class T> __device__ __forceinline__ M3<T> exp_su3(const su3alg<T> &y);
+
+// Step matrix of the learned Metropolis proposal.  FP64 links use exp_su3.
+// FP32 links evaluate the invariants and the series in FP32 when
+// ||H||_F^2 <= 4 (the FP64 closed form otherwise): the proposal only has to be
+// an SU(3) matrix drawn symmetrically, which FP32 rounding of X(w) and X(-w)
+// keeps to the same order as the RST generator's; links are reunitarised
+// after every sweep.
+template <class T> __device__ __forceinline__ M3<T> exp_su3_proposal(const su3alg<T> &y)
+{
+    if constexpr (sizeof(T) == 8) {
+        return exp_su3(y);
+    } else {
+        const float a0 = y.a0, a1 = y.a1, a2 = -(a0 + a1);
+        const float h01r = y.p[1], h01i = -y.p[0], h02r = y.p[3], h02i = -y.p[2], h12r = y.p[5], h12i = -y.p[4];
+        const float n01 = fmaf(h01r, h01r, h01i * h01i), n02 = fmaf(h02r, h02r, h02i * h02i),
+                    n12 = fmaf(h12r, h12r, h12i * h12i);
+        const float c1 = 0.5f * (a0 * a0 + a1 * a1 + a2 * a2) + n01 + n02 + n12;
+        if (c1 > 2.0f) return exp_su3(y);
+        const float tr = h01r * h12r - h01i * h12i, ti = h01r * h12i + h01i * h12r;
+        const float c0 = a0 * a1 * a2 + 2.f * (tr * h02r + ti * h02i) - a0 * n12 - a1 * n02 - a2 * n01;
+        float f[3][2];
+        exp_coefficients_series_f32<16>(c0, c1, f);
+        return exp_assemble<float>(expand(y), f[0][0], f[0][1], f[1][0], f[1][1], f[2][0], f[2][1]);
+    }
+}
+
 template <class T> __device__ __forceinline__ M3<T> exp_su3(const su3alg<T> &y)
 {
     // H = -iY: H_jj = a_j, H_jk = -i Y_jk (j < k)
@@ -128,25 +199,7 @@ template <class T> __device__ __forceinline__ M3<T> exp_su3(const su3alg<T> &y)
     else
         f = exp_coefficients(c0, c1);
     // H = -iY, H^2 = -Y^2:  X = f0 1 + f1 (-iY) + f2 (-Y^2)
-    const M3<T> Y2 = square_antiherm(Y);
-    const T f1r = T(f.f[1][0]), f1i = T(f.f[1][1]), f2r = T(f.f[2][0]), f2i = T(f.f[2][1]);
-    M3<T> x;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            // f1 * (-i Y) = (f1r + i f1i)(Y.i - i Y.r) ;  -f2 Y2
-            const T hr = Y.e[i][j].i, hi = -Y.e[i][j].r;
-            x.e[i][j].r = fma(f1r, hr, fma(-f1i, hi, fma(-f2r, Y2.e[i][j].r, f2i * Y2.e[i][j].i)));
-            x.e[i][j].i = fma(f1r, hi, fma(f1i, hr, fma(-f2r, Y2.e[i][j].i, -f2i * Y2.e[i][j].r)));
-        }
-    x.e[0][0].r += T(f.f[0][0]);
-    x.e[0][0].i += T(f.f[0][1]);
-    x.e[1][1].r += T(f.f[0][0]);
-    x.e[1][1].i += T(f.f[0][1]);
-    x.e[2][2].r += T(f.f[0][0]);
-    x.e[2][2].i += T(f.f[0][1]);
-    return x;
+    return exp_assemble<T>(Y, T(f.f[0][0]), T(f.f[0][1]), T(f.f[1][0]), T(f.f[1][1]), T(f.f[2][0]), T(f.f[2][1]));
 }
 
 } // namespace glacx

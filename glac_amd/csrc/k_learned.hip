@@ -353,7 +353,7 @@ __global__ __launch_bounds__(kBlock) void metropolis_learned_kernel(typename vec
                     for (int b = 0; b <= k; b++) v = fmaf(L[k * (k + 1) / 2 + b], z[b], v);
                     om[k] = v;
                 }
-                xm = exp_su3(algebra_from_omega<T>(om));
+                xm = exp_su3_proposal(algebra_from_omega<T>(om));
             } else {
                 xm = rst_proposal<T>(rs, x0abs, eps);
             }
