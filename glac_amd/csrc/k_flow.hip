@@ -92,11 +92,11 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
                 // the product's third row once (36 multiply-adds fewer per
                 // product than rebuilding both operands).
                 M3<T> p;  // U_nu(x+mu) U_mu(x+nu)^dag
-                mm<false, true, false, 2>(p, load_rows01<T>(vnu, ipm), load_link<T, R>(vmu, ipn));
+                mm<false, true, false, 2>(p, load_rows01<T, R>(vnu, ipm), load_link<T, R>(vmu, ipn));
                 rebuild_third_row(p);
                 mm<false, true, true>(a, p, lds_load(sh.own[nu], lane));
                 M3<T> q;  // U_mu(x-nu) U_nu(x+mu-nu), whose dagger the lower staple uses
-                mm<false, false, false, 2>(q, load_rows01<T>(vmu, imn), load_link<T, R>(vnu, ipmmn));
+                mm<false, false, false, 2>(q, load_rows01<T, R>(vmu, imn), load_link<T, R>(vnu, ipmmn));
                 rebuild_third_row(q);
                 mm<true, false, true>(a, q, lds_load(sh.back[nu], lane));
             } else {

@@ -21,9 +21,10 @@
 // bz Nx Ny / 2 entries, a few hundred KB of links at 32^4, inside an XCD's L2,
 // where plain rows would put it a whole 3D slice away.
 //
-// A field is 36 complex planes (mu-major, then matrix entry row-major), each
-// V entries long: entry (i, j) of U_mu at idx lives at
-// plane[(mu * 9 + 3 i + j) * V + idx].
+// A field is four direction regions of 9 V complex entries (mu-major).  In
+// FP64 each region is 9 planes, entry (i, j) of U_mu at idx at
+// plane[(mu * 9 + 3 i + j) * V + idx]; FP32 pairs entries into float4 planes
+// (see load_rows01 below).
 //
 // GLAC's shift() and Communicator link getters (GLAC/math/lattice.h:824-1144,
 // GLAC/parallelization/communicator.cpp:83-258) become the index arithmetic
@@ -264,20 +265,63 @@ template <class T> __host__ __device__ __forceinline__ void rebuild_third_row(M3
     }
 }
 
+// FP32 links pair their entries: plane pair q holds entries 2q and 2q + 1 of
+// every link as one float4 (entry e = 3 i + j is U_ij), so the first two rows
+// are three 16-byte loads; with three stored rows a fourth pair holds
+// entries 6, 7 and entry 8 follows as a float2 plane (byte offset 64 V).  The
+// direction's region is 72 V bytes either way; compressed links (R = 2) use
+// its first 48 V.  FP64 links keep one double2 plane per entry.
+__device__ __forceinline__ float4 buf_load_pair(const DirView<float> &d, unsigned off16, int q)
+{
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(d.r, off16, q * 2 * d.plane_bytes, 0));
+}
+__device__ __forceinline__ void buf_store_pair(const DirView<float> &d, unsigned off16, int q, float4 z)
+{
+    using U4 = unsigned int __attribute__((ext_vector_type(4)));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, z), d.r, off16, q * 2 * d.plane_bytes, 0);
+}
+
+// First two rows only (row 2 left unset), for products that need no more of
+// this operand.
+template <class T, int R> __device__ __forceinline__ M3<T> load_rows01(const DirView<T> &d, int idx)
+{
+    M3<T> m;
+    if constexpr (sizeof(T) == 4) {
+        const unsigned off = (unsigned)idx * 16u;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const float4 z = buf_load_pair(d, off, q);
+            m.e[(2 * q) / 3][(2 * q) % 3] = {z.x, z.y};
+            m.e[(2 * q + 1) / 3][(2 * q + 1) % 3] = {z.z, z.w};
+        }
+    } else {
+        using V2 = typename vec2<T>::type;
+        const unsigned off = (unsigned)idx * (unsigned)sizeof(V2);
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            const V2 z = buf_load(d, off, k);
+            m.e[k / 3][k % 3] = {z.x, z.y};
+        }
+    }
+    return m;
+}
+
 template <class T, int R> __device__ __forceinline__ M3<T> load_link(const DirView<T> &d, int idx)
 {
-    using V2 = typename vec2<T>::type;
-    const unsigned off = (unsigned)idx * (unsigned)sizeof(V2);
-    M3<T> m;
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-        const V2 z = buf_load(d, off, k);
-        m.e[k / 3][k % 3] = {z.x, z.y};
-    }
     static_assert(R == 2 || R == 3, "links store 2 or 3 rows");
-    if (R == 2) {
+    static_assert(R == 3 || sizeof(T) == 4, "compressed links are FP32");
+    M3<T> m = load_rows01<T, R>(d, idx);
+    if constexpr (R == 2) {
         rebuild_third_row(m);
+    } else if constexpr (sizeof(T) == 4) {
+        const float4 z = buf_load_pair(d, (unsigned)idx * 16u, 3);
+        m.e[2][0] = {z.x, z.y};
+        m.e[2][1] = {z.z, z.w};
+        const float2 w = buf_load(d, (unsigned)idx * 8u, 8);
+        m.e[2][2] = {w.x, w.y};
     } else {
+        using V2 = typename vec2<T>::type;
+        const unsigned off = (unsigned)idx * (unsigned)sizeof(V2);
 #pragma unroll
         for (int k = 6; k < 9; k++) {
             const V2 z = buf_load(d, off, k);
@@ -287,31 +331,26 @@ template <class T, int R> __device__ __forceinline__ M3<T> load_link(const DirVi
     return m;
 }
 
-// The first two rows only (row 2 left unset), for products that need no more
-// of this operand.
-template <class T> __device__ __forceinline__ M3<T> load_rows01(const DirView<T> &d, int idx)
-{
-    using V2 = typename vec2<T>::type;
-    const unsigned off = (unsigned)idx * (unsigned)sizeof(V2);
-    M3<T> m;
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-        const V2 z = buf_load(d, off, k);
-        m.e[k / 3][k % 3] = {z.x, z.y};
-    }
-    return m;
-}
-
 template <class T, int R> __device__ __forceinline__ void store_link(const DirView<T> &d, int idx, const M3<T> &m)
 {
-    using V2 = typename vec2<T>::type;
-    const unsigned off = (unsigned)idx * (unsigned)sizeof(V2);
+    if constexpr (sizeof(T) == 4) {
+        const unsigned off = (unsigned)idx * 16u;
 #pragma unroll
-    for (int k = 0; k < 3 * R; k++) {
-        V2 z;
-        z.x = m.e[k / 3][k % 3].r;
-        z.y = m.e[k / 3][k % 3].i;
-        buf_store(d, off, k, z);
+        for (int q = 0; q < (R == 2 ? 3 : 4); q++) {
+            const cpx<T> a = m.e[(2 * q) / 3][(2 * q) % 3], b = m.e[(2 * q + 1) / 3][(2 * q + 1) % 3];
+            buf_store_pair(d, off, q, make_float4(a.r, a.i, b.r, b.i));
+        }
+        if constexpr (R == 3) buf_store(d, (unsigned)idx * 8u, 8, make_float2(m.e[2][2].r, m.e[2][2].i));
+    } else {
+        using V2 = typename vec2<T>::type;
+        const unsigned off = (unsigned)idx * (unsigned)sizeof(V2);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            V2 z;
+            z.x = m.e[k / 3][k % 3].r;
+            z.y = m.e[k / 3][k % 3].i;
+            buf_store(d, off, k, z);
+        }
     }
 }
 
@@ -332,13 +371,23 @@ template <class T, int R>
 __device__ __forceinline__ void store_link_any(typename vec2<T>::type *__restrict__ f, const Geom &g, int mu, int idx,
                                                const M3<T> &m)
 {
-    typename vec2<T>::type *p = f + (size_t)mu * 9 * g.v + idx;
+    typename vec2<T>::type *p = f + (size_t)mu * 9 * g.v;
+    if constexpr (sizeof(T) == 4) {
+        float4 *q4 = reinterpret_cast<float4 *>(p) + idx;
 #pragma unroll
-    for (int k = 0; k < 3 * R; k++) {
-        typename vec2<T>::type z;
-        z.x = m.e[k / 3][k % 3].r;
-        z.y = m.e[k / 3][k % 3].i;
-        p[(size_t)k * g.v] = z;
+        for (int q = 0; q < (R == 2 ? 3 : 4); q++) {
+            const cpx<T> a = m.e[(2 * q) / 3][(2 * q) % 3], b = m.e[(2 * q + 1) / 3][(2 * q + 1) % 3];
+            q4[(size_t)q * g.v] = make_float4(a.r, a.i, b.r, b.i);
+        }
+        if constexpr (R == 3) p[(size_t)8 * g.v + idx] = make_float2(m.e[2][2].r, m.e[2][2].i);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            typename vec2<T>::type z;
+            z.x = m.e[k / 3][k % 3].r;
+            z.y = m.e[k / 3][k % 3].i;
+            p[(size_t)k * g.v + idx] = z;
+        }
     }
 }
 
