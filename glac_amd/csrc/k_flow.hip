@@ -14,7 +14,7 @@
 //   stage 2:  Y = 3/4 eps Z2 - acc;                      W3 = exp(Y) W2
 //
 // The accumulator holds the 8 real parameters of an su(3) element per link
-// (8 planes of T per direction).
+// (16-byte vectors, see acc_store).
 //
 // FP32 links are reunitarised (FP64 Gram-Schmidt) as they are stored.
 //
@@ -53,6 +53,44 @@ template <class T> __device__ __forceinline__ M3<T> lds_load(const T (&s)[18][kF
 #pragma unroll
     for (int e = 0; e < 9; e++) m.e[e / 3][e % 3] = {s[2 * e][lane], s[2 * e + 1][lane]};
     return m;
+}
+
+// RK3 accumulator: the 8 real parameters (a0, a1, p0..p5) of an su(3)
+// element per link, in 16-byte vectors: two float4 planes per direction in
+// FP32 (a0 a1 p0 p1 | p2 p3 p4 p5), four double2 planes in FP64.
+__device__ __forceinline__ void acc_store(float *acc, const Geom &g, int mu, int idx, const su3alg<float> &z)
+{
+    float4 *q = reinterpret_cast<float4 *>(acc) + (size_t)mu * 2 * g.v + idx;
+    q[0] = make_float4(z.a0, z.a1, z.p[0], z.p[1]);
+    q[(size_t)g.v] = make_float4(z.p[2], z.p[3], z.p[4], z.p[5]);
+}
+__device__ __forceinline__ su3alg<float> acc_load(const float *acc, const Geom &g, int mu, int idx)
+{
+    const float4 *q = reinterpret_cast<const float4 *>(acc) + (size_t)mu * 2 * g.v + idx;
+    const float4 u = q[0], v = q[(size_t)g.v];
+    su3alg<float> z;
+    z.a0 = u.x, z.a1 = u.y, z.p[0] = u.z, z.p[1] = u.w, z.p[2] = v.x, z.p[3] = v.y, z.p[4] = v.z, z.p[5] = v.w;
+    return z;
+}
+__device__ __forceinline__ void acc_store(double *acc, const Geom &g, int mu, int idx, const su3alg<double> &z)
+{
+    double2 *q = reinterpret_cast<double2 *>(acc) + (size_t)mu * 4 * g.v + idx;
+    q[0] = make_double2(z.a0, z.a1);
+#pragma unroll
+    for (int k = 0; k < 3; k++) q[(size_t)(k + 1) * g.v] = make_double2(z.p[2 * k], z.p[2 * k + 1]);
+}
+__device__ __forceinline__ su3alg<double> acc_load(const double *acc, const Geom &g, int mu, int idx)
+{
+    const double2 *q = reinterpret_cast<const double2 *>(acc) + (size_t)mu * 4 * g.v + idx;
+    su3alg<double> z;
+    const double2 u = q[0];
+    z.a0 = u.x, z.a1 = u.y;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double2 v = q[(size_t)(k + 1) * g.v];
+        z.p[2 * k] = v.x, z.p[2 * k + 1] = v.y;
+    }
+    return z;
 }
 
 template <class T, int R, int STAGE>
@@ -116,28 +154,20 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
     if (!valid) return;
     const M3<T> u = lds_load(sh.own[mu], lane);
     const su3alg<T> z = force_from_omega(mul(u, a));
-    T *ap = acc + (size_t)mu * 8 * g.v + idx;
     su3alg<T> y;
     if (STAGE == 0) {
-        ap[0] = z.a0;
-        ap[(size_t)g.v] = z.a1;
-#pragma unroll
-        for (int k = 0; k < 6; k++) ap[(size_t)(k + 2) * g.v] = z.p[k];
+        acc_store(acc, g, mu, idx, z);
         y.a0 = cz * z.a0;
         y.a1 = cz * z.a1;
 #pragma unroll
         for (int k = 0; k < 6; k++) y.p[k] = cz * z.p[k];
     } else {
-        y.a0 = cz * z.a0 - ca * ap[0];
-        y.a1 = cz * z.a1 - ca * ap[(size_t)g.v];
+        const su3alg<T> c = acc_load(acc, g, mu, idx);
+        y.a0 = cz * z.a0 - ca * c.a0;
+        y.a1 = cz * z.a1 - ca * c.a1;
 #pragma unroll
-        for (int k = 0; k < 6; k++) y.p[k] = cz * z.p[k] - ca * ap[(size_t)(k + 2) * g.v];
-        if (STAGE == 1) {
-            ap[0] = y.a0;
-            ap[(size_t)g.v] = y.a1;
-#pragma unroll
-            for (int k = 0; k < 6; k++) ap[(size_t)(k + 2) * g.v] = y.p[k];
-        }
+        for (int k = 0; k < 6; k++) y.p[k] = cz * z.p[k] - ca * c.p[k];
+        if (STAGE == 1) acc_store(acc, g, mu, idx, y);
     }
     M3<T> w = zero3<T>();
     mm<false, false, false, R>(w, exp_su3(y), u);  // two rows suffice when two are stored
