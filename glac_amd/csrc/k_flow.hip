@@ -26,6 +26,7 @@
 // share most staple links -- are processed on the same CU at the same time.
 #include "expsu3.hpp"
 #include "kernels.hpp"
+#include "ldsmat.hpp"
 
 namespace glacx {
 
@@ -34,26 +35,8 @@ namespace glacx {
 // wavefronts take them from here for their staples instead of gathering them
 // again (U_nu(x) closes every upper staple, U_nu(x - nu) every lower one).
 // Per site that is 56 link gathers from memory instead of 76.
-template <class T> struct FlowLds {
-    T own[4][18][kFlowSites];
-    T back[4][18][kFlowSites];
-};
-
-template <class T> __device__ __forceinline__ void lds_store(T (&s)[18][kFlowSites], int lane, const M3<T> &m)
-{
-#pragma unroll
-    for (int e = 0; e < 9; e++) {
-        s[2 * e][lane] = m.e[e / 3][e % 3].r;
-        s[2 * e + 1][lane] = m.e[e / 3][e % 3].i;
-    }
-}
-template <class T> __device__ __forceinline__ M3<T> lds_load(const T (&s)[18][kFlowSites], int lane)
-{
-    M3<T> m;
-#pragma unroll
-    for (int e = 0; e < 9; e++) m.e[e / 3][e % 3] = {s[2 * e][lane], s[2 * e + 1][lane]};
-    return m;
-}
+// slots 0..3: U_mu(x) for mu = 0..3, slots 4..7: U_mu(x - mu)
+template <class T> using FlowLds = LdsMats<T, 8>;
 
 // RK3 accumulator: the 8 real parameters (a0, a1, p0..p5) of an su(3)
 // element per link, in 16-byte vectors: two float4 planes per direction in
@@ -107,8 +90,8 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
     const int idx = parity_index(g, lane >> 5, h);
     const Coord x = decode(g, idx);
     const DirView<T> vmu = dir_view<T>(win, g, mu);
-    lds_store(sh.own[mu], lane, load_link<T, R>(vmu, idx));
-    lds_store(sh.back[mu], lane, load_link<T, R>(vmu, index_of(g, step(g, x, mu, -1))));
+    lds_put(sh, mu, lane, load_link<T, R>(vmu, idx));
+    lds_put(sh, 4 + mu, lane, load_link<T, R>(vmu, index_of(g, step(g, x, mu, -1))));
     __syncthreads();
     // staple A_mu(x) = sum_{nu != mu} [ U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
     //                                 + U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu) ]
@@ -132,27 +115,27 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
                 M3<T> p;  // U_nu(x+mu) U_mu(x+nu)^dag
                 mm<false, true, false, 2>(p, load_rows01<T, R>(vnu, ipm), load_link<T, R>(vmu, ipn));
                 rebuild_third_row(p);
-                mm<false, true, true>(a, p, lds_load(sh.own[nu], lane));
+                mm<false, true, true>(a, p, lds_get(sh, nu, lane));
                 M3<T> q;  // U_mu(x-nu) U_nu(x+mu-nu), whose dagger the lower staple uses
                 mm<false, false, false, 2>(q, load_rows01<T, R>(vmu, imn), load_link<T, R>(vnu, ipmmn));
                 rebuild_third_row(q);
-                mm<true, false, true>(a, q, lds_load(sh.back[nu], lane));
+                mm<true, false, true>(a, q, lds_get(sh, 4 + nu, lane));
             } else {
                 {
                     const M3<T> u1 = load_link<T, R>(vnu, ipm);
                     const M3<T> u2 = load_link<T, R>(vmu, ipn);
-                    mm<false, true, true>(a, mul_ad(u1, u2), lds_load(sh.own[nu], lane));
+                    mm<false, true, true>(a, mul_ad(u1, u2), lds_get(sh, nu, lane));
                 }
                 {
                     const M3<T> u1 = load_link<T, R>(vnu, ipmmn);
                     const M3<T> u2 = load_link<T, R>(vmu, imn);
-                    mm<false, false, true>(a, mul_dd(u1, u2), lds_load(sh.back[nu], lane));
+                    mm<false, false, true>(a, mul_dd(u1, u2), lds_get(sh, 4 + nu, lane));
                 }
             }
         }
     }
     if (!valid) return;
-    const M3<T> u = lds_load(sh.own[mu], lane);
+    const M3<T> u = lds_get(sh, mu, lane);
     const su3alg<T> z = force_from_omega(mul(u, a));
     su3alg<T> y;
     if (STAGE == 0) {
