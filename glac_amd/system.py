@@ -87,7 +87,7 @@ def run_ensemble(L: Lattice, cfg: EnsembleConfig, thermalise=True, sweep_offset=
         t1 = time.perf_counter()
         sweep += cfg.NCor
         L.copy(FLOW, MAIN)
-        L.flow(cfg.NFlows, cfg.flowEpsilon, field=FLOW)
+        L.flow(cfg.NFlows, cfg.flowEpsilon, field=FLOW, measure=False)
         p_chain = L.plaquette(MAIN)
         p, e_raw, qq, _ = L.measure(FLOW)
         t2 = time.perf_counter()
