@@ -23,15 +23,15 @@ def banner(msg):
 
 @pytest.fixture(scope="module")
 def flows(tmp_path_factory):
-    path = str(tmp_path_factory.mktemp("cfg") / "cfg2.bin")
-    banner(f"config-2 thermalise {N}")
+    path = str(tmp_path_factory.mktemp("cfg") / "thermalised.bin")
+    banner(f"thermalise {N}")
     with glacx.Lattice(N, glacx.FP64) as L:
         L.hot_start(6)
         L.metropolis(500, 6.0, 0.24, 10, seed=6)
         L.save_bin(path)
     out = {}
     for prec in (glacx.FP64, glacx.FP32_LINKS, glacx.FP32_COMPRESSED):
-        banner(f"config-2 flow 1000 steps precision {prec}")
+        banner(f"flow {N} 1000 steps precision {prec}")
         with glacx.Lattice(N, prec) as L:
             L.load_bin(path, field=glacx.FLOW)
             out[prec] = L.flow(1000, 0.01, field=glacx.FLOW)

@@ -153,11 +153,11 @@ def test_flow_matches_oracle_per_step(n, steps, spread, prec):
         assert np.abs(q - wq).max() < 1e-4
 
 
-def test_flow_config1_shape_against_oracle(prec):
-    """8^3 x 16 (the shape of GLAC's CPU config): a rough Metropolis-like
-    field flowed 100 RK3 steps at eps = 0.01; E(t) and Q(t) per step."""
+def test_flow_8x8x8x16_against_oracle(prec):
+    """8^3 x 16: a rough Metropolis-like field flowed 100 RK3 steps at
+    eps = 0.01; E(t) and Q(t) per step."""
     n = (8, 8, 8, 16)
-    banner(f"flow config-1 shape {n} 100 steps {PNAME[prec]}")
+    banner(f"flow {n} 100 steps {PNAME[prec]}")
     U = O.hot_start_haar(n, 99)
     for s in range(4):
         O.cb_sweep(U, n, 6.0, 0.24, 10, 99, s)
