@@ -66,7 +66,7 @@ __host__ __device__ inline int fdiv(int n, const FastDiv &f) { return (int)(((ui
 struct Geom {
     int n[4];       // Nx, Ny, Nz, Nt
     int nxh;        // Nx / 2
-    int lbz, lbt;   // log2 of the block extents bz, bt (1 or 2)
+    int lbz, lbt;   // log2 of the block extents bz, bt
     int nzb;        // z-blocks, Nz / bz
     int run;        // entries of one (block, parity, lt): bz Nx Ny / 2
     int bh;         // entries of one (block, parity): bt run
@@ -77,13 +77,22 @@ struct Geom {
     FastDiv d_nxh, d_ny, d_nzb, d_run, d_bh, d_2bh, d_cpr;
 };
 
+// log2 of the preferred z and t block extents (a build-time tuning knob);
+// an extent not divisible by the preferred block falls back to blocks of 2.
+#ifndef GLACX_BLOCK_LOG_Z
+#define GLACX_BLOCK_LOG_Z 2
+#endif
+#ifndef GLACX_BLOCK_LOG_T
+#define GLACX_BLOCK_LOG_T 2
+#endif
+
 __host__ inline Geom make_geom(const int dims[4])
 {
     Geom g;
     for (int k = 0; k < 4; k++) g.n[k] = dims[k];
     g.nxh = dims[0] / 2;
-    g.lbz = (dims[2] % 4 == 0) ? 2 : 1;
-    g.lbt = (dims[3] % 4 == 0) ? 2 : 1;
+    g.lbz = (dims[2] % (1 << GLACX_BLOCK_LOG_Z) == 0) ? GLACX_BLOCK_LOG_Z : 1;
+    g.lbt = (dims[3] % (1 << GLACX_BLOCK_LOG_T) == 0) ? GLACX_BLOCK_LOG_T : 1;
     g.nzb = dims[2] >> g.lbz;
     g.run = (1 << g.lbz) * dims[1] * g.nxh;
     g.bh = g.run << g.lbt;
