@@ -33,3 +33,15 @@ def test_library_exports_every_declared_symbol():
 
 def test_python_binding_covers_the_abi():
     glac_amd.library()  # sets argtypes for every entry point; raises if one is absent
+
+
+def test_create_rejects_bad_shapes_before_touching_a_device():
+    """Extent and size checks come before any HIP call, so they hold on a
+    machine without a GPU: odd or too small extents, unknown precision, and
+    lattices whose direction region would exceed a 32-bit buffer range."""
+    lib = glac_amd.library()
+    h = ctypes.c_void_p()
+    for dims, prec in [((3, 4, 4, 4), 0), ((4, 4, 4, 0), 0), ((4, 4, 4, 4), 7),
+                       ((80, 80, 80, 80), 0), ((96, 96, 96, 96), 1), ((514, 2, 2, 2), 0)]:
+        rc = lib.glacx_create((ctypes.c_int * 4)(*dims), prec, 0, ctypes.byref(h))
+        assert rc == -1, (dims, prec, rc)
