@@ -120,6 +120,17 @@ def independent_configs(G, n, beta, seconds):
     out["ratio_err"] = ratio * rel
     out["protocol"] = ("NCor 50 sweeps (10 hits, eps 0.24 fallback/RST), flow 50 x 0.02, Q at t = 1; "
                        "tau_int(Q^2) by Wolff's Gamma method (S = 1.5), errors from its estimate")
+    # the same measurement run for 400 s per variant (committed), for comparison
+    # with this run's shorter, noisier estimate
+    ref = os.path.join(ROOT, "profiles", "indep_configs_%s.json" % "x".join(map(str, n)))
+    if os.path.exists(ref):
+        with open(ref) as f:
+            r = json.load(f)
+        out["long_run"] = {"ratio_learned_fp32_over_plain_fp64": r["ratio_learned_fp32_over_plain_fp64"],
+                           "ratio_err": r["ratio_err"], "seconds_per_variant": 400,
+                           "fp32c_learned_indep_configs_per_s": r["fp32c_learned"]["indep_configs_per_s"],
+                           "fp64_plain_indep_configs_per_s": r["fp64_plain"]["indep_configs_per_s"],
+                           "source": "profiles/" + os.path.basename(ref)}
     return out
 
 
