@@ -74,6 +74,52 @@ __device__ __forceinline__ ExpCoeffs exp_coefficients(double c0, double c1)
     return r;
 }
 
+// The same closed form in FP32 (the learned proposal's step matrix with FP32 links).
+__device__ __forceinline__ void exp_coefficients_f32(float c0, float c1, float fo[3][2])
+{
+    const float c0max = 2.0f * (c1 * (1.0f / 3.0f)) * sqrtf(c1 * (1.0f / 3.0f));
+    float ratio = fabsf(c0) / c0max;
+    ratio = ratio > 1.0f ? 1.0f : ratio;
+    const float theta = acosf(ratio);
+    float st, ct;
+    sincosf(theta * (1.0f / 3.0f), &st, &ct);
+    const float u = sqrtf(c1 * (1.0f / 3.0f)) * ct;
+    const float w = sqrtf(c1) * st;
+    const float uu = u * u, ww = w * w;
+    float su, cu, sw, cw;
+    sincosf(u, &su, &cu);
+    sincosf(w, &sw, &cw);
+    const float xi0 = fabsf(w) > 0.05f ? sw / w : 1.0f - ww * (1.0f / 6.0f) * (1.0f - ww * (1.0f / 20.0f) * (1.0f - ww * (1.0f / 42.0f)));
+    const float c2u = cu * cu - su * su, s2u = 2.0f * su * cu;
+    float h[3][2];
+    {   // h0 = (u^2 - w^2) e^{2iu} + e^{-iu} [8u^2 cos w + 2iu(3u^2 + w^2) xi0]
+        const float br = 8.0f * uu * cw, bi = 2.0f * u * (3.0f * uu + ww) * xi0;
+        h[0][0] = (uu - ww) * c2u + (cu * br + su * bi);
+        h[0][1] = (uu - ww) * s2u + (cu * bi - su * br);
+    }
+    {   // h1 = 2u e^{2iu} - e^{-iu} [2u cos w - i(3u^2 - w^2) xi0]
+        const float br = 2.0f * u * cw, bi = -(3.0f * uu - ww) * xi0;
+        h[1][0] = 2.0f * u * c2u - (cu * br + su * bi);
+        h[1][1] = 2.0f * u * s2u - (cu * bi - su * br);
+    }
+    {   // h2 = e^{2iu} - e^{-iu} [cos w + 3iu xi0]
+        const float br = cw, bi = 3.0f * u * xi0;
+        h[2][0] = c2u - (cu * br + su * bi);
+        h[2][1] = s2u - (cu * bi - su * br);
+    }
+    const float inv = 1.0f / (9.0f * uu - ww);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        fo[j][0] = h[j][0] * inv;
+        fo[j][1] = h[j][1] * inv;
+    }
+    if (c0 < 0.0f) {  // f_j(-c0) = (-1)^j conj(f_j(c0))
+        fo[0][1] = -fo[0][1];
+        fo[1][0] = -fo[1][0];
+        fo[2][1] = -fo[2][1];
+    }
+}
+
 // inverse factorials 1/n!, n = 0..20
 __device__ constexpr double kInvFact[21] = {
     1.0, 1.0, 0.5, 1.6666666666666666e-01, 4.1666666666666664e-02, 8.3333333333333332e-03,
@@ -151,11 +197,10 @@ template <class T> __device__ __forceinline__ M3<T> exp_assemble(const M3<T> &Y,
 template <class T> __device__ __forceinline__ M3<T> exp_su3(const su3alg<T> &y);
 
 // Step matrix of the learned Metropolis proposal.  FP64 links use exp_su3.
-// FP32 links evaluate the invariants and the series in FP32 when
-// ||H||_F^2 <= 4 (the FP64 closed form otherwise): the proposal only has to be
-// an SU(3) matrix drawn symmetrically, which FP32 rounding of X(w) and X(-w)
-// keeps to the same order as the RST generator's; links are reunitarised
-// after every sweep.
+// FP32 links evaluate everything in FP32: the series when ||H||_F^2 <= 4, the
+// closed form otherwise.  The proposal only has to be an SU(3) matrix drawn
+// symmetrically, which FP32 rounding of X(w) and X(-w) keeps to the same
+// order as the RST generator's; links are reunitarised after every sweep.
 template <class T> __device__ __forceinline__ M3<T> exp_su3_proposal(const su3alg<T> &y)
 {
     if constexpr (sizeof(T) == 8) {
@@ -166,11 +211,13 @@ template <class T> __device__ __forceinline__ M3<T> exp_su3_proposal(const su3al
         const float n01 = fmaf(h01r, h01r, h01i * h01i), n02 = fmaf(h02r, h02r, h02i * h02i),
                     n12 = fmaf(h12r, h12r, h12i * h12i);
         const float c1 = 0.5f * (a0 * a0 + a1 * a1 + a2 * a2) + n01 + n02 + n12;
-        if (c1 > 2.0f) return exp_su3(y);
         const float tr = h01r * h12r - h01i * h12i, ti = h01r * h12i + h01i * h12r;
         const float c0 = a0 * a1 * a2 + 2.f * (tr * h02r + ti * h02i) - a0 * n12 - a1 * n02 - a2 * n01;
         float f[3][2];
-        exp_coefficients_series_f32<16>(c0, c1, f);
+        if (c1 <= 2.0f)
+            exp_coefficients_series_f32<16>(c0, c1, f);
+        else
+            exp_coefficients_f32(c0, c1, f);
         return exp_assemble<float>(expand(y), f[0][0], f[0][1], f[1][0], f[1][1], f[2][0], f[2][1]);
     }
 }
