@@ -9,7 +9,7 @@
 //       FP32 accumulation;
 //   output o -> lower-triangular L: L_aa = exp(o_a), L_ab = o[8 + a(a-1)/2 + b];
 //   per hit w = L z with z symmetric (z_a = (c1 + c2) sqrt 6, c uniform on the
-//       2^24 midpoints of [-1/2, 1/2)), X = exp(i sum_a w_a lambda_a / 2),
+//       2^16 midpoints of [-1/2, 1/2), two per 32-bit output), X = exp(i sum_a w_a lambda_a / 2),
 //       U' = X U, accepted with GLAC's exp(-dS) > u.
 // L depends on A only and z -> -z is a symmetry of the draw, so the
 // proposal density of X equals that of X^dagger: exact accept/reject keeps
@@ -126,11 +126,14 @@ template <class T> __device__ __forceinline__ su3alg<T> algebra_from_omega(const
     return y;
 }
 
-// symmetric unit-variance draw from a stream (c uniform on the 2^24 midpoints of [-1/2, 1/2))
+// symmetric unit-variance draw from one 32-bit stream output: (c1 + c2) sqrt(6)
+// with c1 (high 16 bits) and c2 (low 16 bits) on the 2^16 midpoints of
+// [-1/2, 1/2), exact in FP32
 template <class S> __device__ __forceinline__ float sym_draw(S &rs)
 {
-    const float c1 = (float)(rs.next() >> 8) * 0x1p-24f - (0.5f - 0x1p-25f);
-    const float c2 = (float)(rs.next() >> 8) * 0x1p-24f - (0.5f - 0x1p-25f);
+    const uint32_t b = rs.next();
+    const float c1 = (float)(b >> 16) * 0x1p-16f - (0.5f - 0x1p-17f);
+    const float c2 = (float)(b & 0xFFFFu) * 0x1p-16f - (0.5f - 0x1p-17f);
     return (c1 + c2) * 2.4494897427831781f;
 }
 

@@ -1006,12 +1006,14 @@ void or_project_algebra(const double *N, double g[8])
     g[7] = -0.5 * r3 * (IM(N, 0, 0) + IM(N, 1, 1) - 2.0 * IM(N, 2, 2));
 }
 
-/* symmetric unit-variance draw: (c1 + c2) sqrt(6) with c uniform on the
- * 2^24 midpoints of [-1/2, 1/2) -- exactly symmetric about 0 */
+/* symmetric unit-variance draw from one 32-bit output: (c1 + c2) sqrt(6)
+ * with c1 (high 16 bits) and c2 (low 16 bits) uniform on the 2^16 midpoints
+ * of [-1/2, 1/2) -- exactly symmetric about 0 */
 static inline double stream_sym(uint32_t s[4])
 {
-    const double c1 = (double)(or_xoshiro128ss_next(s) >> 8) * 0x1p-24 - (0.5 - 0x1p-25);
-    const double c2 = (double)(or_xoshiro128ss_next(s) >> 8) * 0x1p-24 - (0.5 - 0x1p-25);
+    const uint32_t b = or_xoshiro128ss_next(s);
+    const double c1 = (double)(b >> 16) * 0x1p-16 - (0.5 - 0x1p-17);
+    const double c2 = (double)(b & 0xFFFFu) * 0x1p-16 - (0.5 - 0x1p-17);
     return (c1 + c2) * 2.449489742783178;
 }
 
