@@ -20,11 +20,13 @@ import numpy as np
 
 __all__ = ["Lattice", "GlacxError", "library", "FP64", "FP32_LINKS", "FP32_COMPRESSED", "MAIN", "FLOW",
            "K_FLOW_STAGE", "K_MEASURE", "K_METROPOLIS", "K_PLAQUETTE", "K_METROPOLIS_LEARNED", "K_TRAIN",
+           "K_FLOW_STAGE_MEASURE",
            "PROPOSAL_NPARAM", "LIB_PATH"]
 
 FP64, FP32_LINKS, FP32_COMPRESSED = 0, 1, 2
 MAIN, FLOW = 0, 1
 K_FLOW_STAGE, K_MEASURE, K_METROPOLIS, K_PLAQUETTE, K_METROPOLIS_LEARNED, K_TRAIN = 0, 1, 2, 3, 4, 5
+K_FLOW_STAGE_MEASURE = 6  # flow stage 0 with the clover measurement fused
 PROPOSAL_NPARAM = 2468
 
 _HERE = os.path.dirname(os.path.abspath(__file__))

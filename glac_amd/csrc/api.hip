@@ -179,19 +179,50 @@ template <class T, int R> int run_plaquette(glacx_ctx *c, const void *f, double 
     return GLACX_OK;
 }
 
-// One RK3 step on `f` (result back in `f`): three stages ping-ponging
-// through the work buffer, then a device copy back for the odd stage count.
-template <class T, int R> int run_flow_step(glacx_ctx *c, void *f, double eps)
+// One RK3 step on `f`: three stages ping-ponging through the work buffer
+// (the caller swaps the roles of the two afterwards).  With `meas` stage 0
+// also measures the clover observables of `f` into the measurement partials.
+template <class T, int R> int run_flow_step(glacx_ctx *c, void *f, double eps, double *meas)
 {
     void *bufs[2] = {f, c->work};
     int rc;
     for (int s = 0; s < 3; s++) {
         hipEvent_t e0 = nullptr;
         if ((rc = timed_begin(c, &e0))) return rc;
-        launch_flow_stage<T, R>(bufs[s & 1], bufs[(s + 1) & 1], c->acc, c->geom, s, eps, c->stream);
+        launch_flow_stage<T, R>(bufs[s & 1], bufs[(s + 1) & 1], c->acc, c->geom, s, eps, s == 0 ? meas : nullptr,
+                                c->stream);
         CHECK_LAUNCH(c);
-        if ((rc = timed_end(c, GLACX_K_FLOW_STAGE, e0))) return rc;
+        if ((rc = timed_end(c, s == 0 && meas ? GLACX_K_FLOW_STAGE_MEASURE : GLACX_K_FLOW_STAGE, e0))) return rc;
     }
+    return GLACX_OK;
+}
+
+// n_steps flow steps of `field`, measuring before the first and after every
+// step into results (rows of `row` doubles) when results is set.  FP32 links
+// measure inside each step's stage 0 (the field it reads is the previous
+// step's result) and once more after the last step; FP64 links, the
+// SuperSampler set and lattices whose runs are not whole 32-site chunks use
+// the clover kernel between steps.
+template <class T, int R>
+int run_flow(glacx_ctx *c, int field, int n_steps, double eps, double *results, size_t row, bool super)
+{
+    const bool fuse = results && !super && sizeof(T) == 4 && flow_measure_fusable(c->geom);
+    const int mode = super ? 1 : 0;
+    int rc;
+    if (fuse && (rc = ensure(c, &c->partials, &c->partials_len, (size_t)measure_partials_count(c->geom)))) return rc;
+    if (results && !fuse && (rc = run_measure<T, R>(c, c->field[field], results, mode))) return rc;
+    for (int i = 0; i < n_steps; i++) {
+        if ((rc = run_flow_step<T, R>(c, c->field[field], eps, fuse ? c->partials : nullptr))) return rc;
+        if (fuse) {
+            launch_measure_finish(c->partials, c->geom, results + row * (size_t)i, false, c->stream);
+            CHECK_LAUNCH(c);
+        }
+        // three stages leave the result in the work buffer: swap roles
+        std::swap(c->field[field], c->work);
+        if (results && !fuse && (rc = run_measure<T, R>(c, c->field[field], results + row * (size_t)(i + 1), mode)))
+            return rc;
+    }
+    if (fuse && (rc = run_measure<T, R>(c, c->field[field], results + row * (size_t)n_steps, mode))) return rc;
     return GLACX_OK;
 }
 
@@ -630,15 +661,7 @@ int flow_impl(glacx_ctx *c, int field, int n_steps, double eps, bool super, doub
     const size_t row = super ? (size_t)4 + 2 * nt : (size_t)3 + nt;
     int rc;
     if (meas && (rc = ensure(c, &c->results, &c->results_len, row * (size_t)(n_steps + 1)))) return rc;
-    void *f = c->field[field];
-    if (meas && (rc = DISPATCH(c, run_measure, c, f, c->results, super ? 1 : 0))) return rc;
-    for (int i = 0; i < n_steps; i++) {
-        if ((rc = DISPATCH(c, run_flow_step, c, f, eps))) return rc;
-        // three stages leave the result in the work buffer: swap roles
-        std::swap(c->field[field], c->work);
-        f = c->field[field];
-        if (meas && (rc = DISPATCH(c, run_measure, c, f, c->results + row * (size_t)(i + 1), super ? 1 : 0))) return rc;
-    }
+    if ((rc = DISPATCH(c, run_flow, c, field, n_steps, eps, meas ? c->results : nullptr, row, super))) return rc;
     if (meas) {
         std::vector<double> h(row * (size_t)(n_steps + 1));
         HIP_TRY(c, hipMemcpyAsync(h.data(), c->results, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));

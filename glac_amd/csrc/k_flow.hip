@@ -76,15 +76,111 @@ __device__ __forceinline__ su3alg<double> acc_load(const double *acc, const Geom
     return z;
 }
 
-template <class T, int R, int STAGE>
+// ---- clover observables fused into stage 0 -------------------------------
+// Stage 0 of a flow step reads the field the previous step produced, which is
+// the field GLAC measures between steps (GLAC/system.cpp:497-503).  Its staple
+// terms already hold three of the four clover leaves at x: for wavefront mu
+// and term nu, with
+//   up  = U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag,
+//   low = U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu),
+// U_mu(x) up is the leaf L1 of plane (mu, nu), U_mu(x) low its leaf L2 and
+// also L4^dag of plane (nu, mu) (GLAC's leaves, GLAC/observables/
+// mastersamplertopcxyz.cpp:255-328).  The clover C = L1 - L2 + L3 - L4 enters
+// the observables only through F = traceless(C - C^dag) (:331-340), which is
+// linear in the leaves; with T(M) = traceless(M - M^dag) and T(-M) = T(M^dag),
+//   F = T(U_mu(x) (up - low)) + T(L3) + T(U_nu(x) low'),
+// low' being wavefront nu's lower term for mu.  So for each of GLAC's planes
+// (0,1) (0,2) (0,3) (2,3) (3,1) (1,2) the first-index wavefront adds
+// T(U_mu (up - low)), the second-index one T(U_nu low'), and one of the two
+// also T(L3), L3 = U_mu(x-mu)^dag U_nu(x-mu-nu)^dag U_mu(x-mu-nu) U_nu(x-nu)
+// (two more gathers; both backward links are in LDS).  Every wavefront forms
+// one extra product per term, and the L3 leaves are dealt 2, 2, 1, 1.
+//
+// F of a plane is 8 reals {Im F00, Im F11, F01, F02, F12} in one LDS slot
+// per plane.  The three staple terms run as three phases separated by
+// barriers, in a per-wavefront order chosen so that the two wavefronts of a
+// plane reach it in different phases: the earlier one stores its
+// contribution, the later one adds its own to it.  No atomics, and the sum
+// of the two is formed in a fixed order.  The L3 leaves go two per phase.
+// After the last phase wavefront w < 3 evaluates the pair (w, w + 3) =
+// ((0, w+1), (rho, sigma)) per site as the clover kernel does:
+//   E += Re tr(F_a F_a) + Re tr(F_b F_b),  q -= Re tr(F_a F_b),
+// and the plaquette sum takes Re tr L1 of each plane from its first-index
+// wavefront.  The block's sums go to the clover kernel's (chunk, t) slot.
+
+// Slot of GLAC's plane {d1, d2} stored as (d1, d2): 0..2 = (0, nu), 3 = (2,3),
+// 4 = (3,1), 5 = (1,2); (mu, nu) is stored as such when mu = 0 or nu = mu % 3 + 1.
+__device__ __forceinline__ bool plane_first(int mu, int nu) { return mu == 0 || nu == mu % 3 + 1; }
+__device__ __forceinline__ int plane_slot(int d1, int d2) { return d1 == 0 ? d2 - 1 : (d1 == 1 ? 5 : d1 + 1); }
+// Phase schedule, entry e = 3 mu + k for wavefront mu in phase k:
+//   kPhaseNu:  the term nu (4 bits each): mu 0: 1 2 3, mu 1: 2 0 3, mu 2: 0 3 1, mu 3: 0 1 2
+//   kPhaseAdd: the plane's other wavefront stored first (add to the slot)
+//   kPhaseL3:  this wavefront also adds the plane's L3 leaf
+constexpr unsigned long long kPhaseNu = 0x210130302321ull;
+constexpr unsigned kPhaseAdd = 0x936u;  // entries 1 2 4 5 8 11
+constexpr unsigned kPhaseL3 = 0x3a3u;   // entries 0 1 5 7 8 9
+
+// T(op(A) B) as {Im T00, Im T11, T01, T02, T12} (the real diagonal of the
+// product is not needed and not computed).
+template <bool DA, class T> __device__ __forceinline__ void ta2_product(const M3<T> &a, const M3<T> &b, T f[8])
+{
+    M3<T> m;
+    mm<DA, false, false>(m, a, b);
+    const T third = (m.e[0][0].i + m.e[1][1].i + m.e[2][2].i) * T(2.0 / 3.0);
+    f[0] = T(2) * m.e[0][0].i - third;
+    f[1] = T(2) * m.e[1][1].i - third;
+    f[2] = m.e[0][1].r - m.e[1][0].r;
+    f[3] = m.e[0][1].i + m.e[1][0].i;
+    f[4] = m.e[0][2].r - m.e[2][0].r;
+    f[5] = m.e[0][2].i + m.e[2][0].i;
+    f[6] = m.e[1][2].r - m.e[2][1].r;
+    f[7] = m.e[1][2].i + m.e[2][1].i;
+}
+
+// Re tr(F G) of two traceless antihermitian matrices in that form
+// (F22 = -(F00 + F11); F_ji = -conj(F_ij)).
+template <class T> __device__ __forceinline__ T retrace_ta2(const T f[8], const T g[8])
+{
+    T s = f[0] * g[0];
+    s = fma(f[1], g[1], s);
+    s = fma(f[0] + f[1], g[0] + g[1], s);
+    T o = f[2] * g[2];
+#pragma unroll
+    for (int k = 3; k < 8; k++) o = fma(f[k], g[k], o);
+    return -s - T(2) * o;
+}
+
+// SU(3) product from the first two rows of `a` (all that load_rows01 reads):
+// the product's first two rows, then its third rebuilt from them.
+template <bool DB, class T> __device__ __forceinline__ M3<T> su3_product(const M3<T> &a, const M3<T> &b)
+{
+    M3<T> p;
+    mm<false, DB, false, 2>(p, a, b);
+    rebuild_third_row(p);
+    return p;
+}
+
+template <class T> struct FlowMeasLds {
+    T f[6][8][kFlowSites];
+};
+template <class T> __device__ __forceinline__ FlowMeasLds<T> &flow_meas_lds()
+{
+    __shared__ FlowMeasLds<T> s;
+    return s;
+}
+
+template <class T, int R, int STAGE, bool MEAS>
 __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<T>::type *__restrict__ win,
                                                             typename vec2<T>::type *__restrict__ wout,
-                                                            T *__restrict__ acc, Geom g, T cz, T ca)
+                                                            T *__restrict__ acc, Geom g, T cz, T ca,
+                                                            double *__restrict__ meas)
 {
+    static_assert(!MEAS || (STAGE == 0 && sizeof(T) == 4), "measurement is fused into FP32 stage 0 only");
     __shared__ FlowLds<T> sh;
     const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // one direction per wavefront
     const int lane = threadIdx.x & 63;
-    const int h_raw = xcd_block(blockIdx.x, gridDim.x) * (kFlowSites / 2) + (lane & 31);
+    const int hb = xcd_block(blockIdx.x, gridDim.x);
+    const int h_raw = hb * (kFlowSites / 2) + (lane & 31);
     const bool valid = h_raw < g.vh;
     const int h = valid ? h_raw : g.vh - 1;  // idle lanes of a partial block mirror a real site, store nothing
     const int idx = parity_index(g, lane >> 5, h);
@@ -92,6 +188,7 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
     const DirView<T> vmu = dir_view<T>(win, g, mu);
     lds_put(sh, mu, lane, load_link<T, R>(vmu, idx));
     lds_put(sh, 4 + mu, lane, load_link<T, R>(vmu, index_of(g, step(g, x, mu, -1))));
+    T plaq = T(0);
     __syncthreads();
     // staple A_mu(x) = sum_{nu != mu} [ U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
     //                                 + U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu) ]
@@ -101,13 +198,54 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
         const Coord xpm = step(g, x, mu, +1);
         const int ipm = index_of(g, xpm);
 #pragma unroll 1
-        for (int d = 1; d < 4; d++) {
-            const int nu = (mu + d) & 3;
+        for (int k = 0; k < 3; k++) {
+            const int e = 3 * mu + k;
+            const int nu = MEAS ? (int)((kPhaseNu >> (4 * e)) & 0xF) : (mu + k + 1) & 3;
             const DirView<T> vnu = dir_view<T>(win, g, nu);
             const int ipn = index_of(g, step(g, x, nu, +1));
             const int imn = index_of(g, step(g, x, nu, -1));
             const int ipmmn = index_of(g, step(g, xpm, nu, -1));
-            if constexpr (R == 2) {
+            if constexpr (MEAS) {
+                // the same staple terms, kept apart for the clover leaves
+                const M3<T> p = su3_product<true>(load_rows01<T, R>(vnu, ipm), load_link<T, R>(vmu, ipn));
+                const M3<T> q = su3_product<false>(load_rows01<T, R>(vmu, imn), load_link<T, R>(vnu, ipmmn));
+                M3<T> up, low;
+                mm<false, true, false>(up, p, lds_get(sh, nu, lane));
+                mm<true, false, false>(low, q, lds_get(sh, 4 + nu, lane));
+                add_to(a, up);
+                add_to(a, low);
+                const M3<T> u = lds_get(sh, mu, lane);
+                const bool first = plane_first(mu, nu);
+                const int d1 = first ? mu : nu, d2 = first ? nu : mu;
+                const int slot = plane_slot(d1, d2);
+                T f[8];
+                if (first) {
+                    plaq += retrace_mul(u, up);
+                    sub_from(up, low);
+                    ta2_product<false>(u, up, f);
+                } else {
+                    ta2_product<false>(u, low, f);
+                }
+                if ((kPhaseL3 >> e) & 1) {
+                    // L3 = (U_d2(y) U_d1(x-d1))^dag (U_d1(y) U_d2(x-d2)),  y = x - d1 - d2
+                    const int iy = index_of(g, step(g, step(g, x, d1, -1), d2, -1));
+                    const DirView<T> v1 = first ? vmu : vnu, v2 = first ? vnu : vmu;
+                    const M3<T> m1 = su3_product<false>(load_rows01<T, R>(v1, iy), lds_get(sh, 4 + d2, lane));
+                    const M3<T> m2 = su3_product<false>(load_rows01<T, R>(v2, iy), lds_get(sh, 4 + d1, lane));
+                    T f3[8];
+                    ta2_product<true>(m2, m1, f3);
+#pragma unroll
+                    for (int j = 0; j < 8; j++) f[j] += f3[j];
+                }
+                T(&fs)[8][kFlowSites] = flow_meas_lds<T>().f[slot];
+                if ((kPhaseAdd >> e) & 1) {
+#pragma unroll
+                    for (int j = 0; j < 8; j++) f[j] = fs[j][lane] + f[j];
+                }
+#pragma unroll
+                for (int j = 0; j < 8; j++) fs[j][lane] = f[j];
+                __syncthreads();
+            } else if constexpr (R == 2) {
                 // Two stored rows: form each SU(3) product's first two rows
                 // from operands of which only those are needed, and rebuild
                 // the product's third row once (36 multiply-adds fewer per
@@ -134,36 +272,65 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
             }
         }
     }
-    if (!valid) return;
-    const M3<T> u = lds_get(sh, mu, lane);
-    const su3alg<T> z = force_from_omega(mul(u, a));
-    su3alg<T> y;
-    if (STAGE == 0) {
-        acc_store(acc, g, mu, idx, z);
-        y.a0 = cz * z.a0;
-        y.a1 = cz * z.a1;
+    if (valid) {
+        const M3<T> u = lds_get(sh, mu, lane);
+        const su3alg<T> z = force_from_omega(mul(u, a));
+        su3alg<T> y;
+        if (STAGE == 0) {
+            acc_store(acc, g, mu, idx, z);
+            y.a0 = cz * z.a0;
+            y.a1 = cz * z.a1;
 #pragma unroll
-        for (int k = 0; k < 6; k++) y.p[k] = cz * z.p[k];
-    } else {
-        const su3alg<T> c = acc_load(acc, g, mu, idx);
-        y.a0 = cz * z.a0 - ca * c.a0;
-        y.a1 = cz * z.a1 - ca * c.a1;
+            for (int k = 0; k < 6; k++) y.p[k] = cz * z.p[k];
+        } else {
+            const su3alg<T> c = acc_load(acc, g, mu, idx);
+            y.a0 = cz * z.a0 - ca * c.a0;
+            y.a1 = cz * z.a1 - ca * c.a1;
 #pragma unroll
-        for (int k = 0; k < 6; k++) y.p[k] = cz * z.p[k] - ca * c.p[k];
-        if (STAGE == 1) acc_store(acc, g, mu, idx, y);
+            for (int k = 0; k < 6; k++) y.p[k] = cz * z.p[k] - ca * c.p[k];
+            if (STAGE == 1) acc_store(acc, g, mu, idx, y);
+        }
+        M3<T> w = zero3<T>();
+        mm<false, false, false, R>(w, exp_su3(y), u);  // two rows suffice when two are stored
+        if constexpr (sizeof(T) == 4) {
+            // FP32 links: project back onto SU(3) in FP64 every stage.  Without
+            // it the rounding of exp(Y) W drifts the links off the group
+            // (|U U^dag - 1| ~ 1e-4 after 1000 steps at 24^3 x 48), and that drift,
+            // not the FP32 measurement, dominates the error of E(t) at large t.
+            M3<double> d = convert<double>(w);
+            reunitarize<R>(d);
+            w = convert<T>(d);
+        }
+        store_link<T, R>(wout, g, mu, idx, w);
     }
-    M3<T> w = zero3<T>();
-    mm<false, false, false, R>(w, exp_su3(y), u);  // two rows suffice when two are stored
-    if constexpr (sizeof(T) == 4) {
-        // FP32 links: project back onto SU(3) in FP64 every stage.  Without
-        // it the rounding of exp(Y) W drifts the links off the group
-        // (|U U^dag - 1| ~ 1e-4 after 1000 steps at 24^3 x 48), and that drift,
-        // not the FP32 measurement, dominates the error of E(t) at large t.
-        M3<double> d = convert<double>(w);
-        reunitarize<R>(d);
-        w = convert<T>(d);
+    if constexpr (MEAS) {
+        // every lane is a real site here (the launch requires whole chunks);
+        // the last phase's barrier has completed every slot
+        double s[kMeasAcc] = {(double)plaq, 0.0, 0.0, 0.0};
+        if (mu < 3) {
+            T fa[8], fb[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                fa[k] = flow_meas_lds<T>().f[mu][k][lane];
+                fb[k] = flow_meas_lds<T>().f[mu + 3][k][lane];
+            }
+            s[1] = retrace_ta2(fa, fa) + retrace_ta2(fb, fb);
+            s[2] = -retrace_ta2(fa, fb);
+        }
+        block_sum<kMeasAcc>(s);
+        if (threadIdx.x == 0) {
+            // the block's 32 sites per parity are chunk jc of the run (blk, lt)
+            const int h0 = hb * (kFlowSites / 2);
+            const int blk = fdiv(h0, g.d_bh);
+            const int rem = h0 - blk * g.bh;
+            const int lt = fdiv(rem, g.d_run);
+            const int jc = (rem - lt * g.run) >> 5;
+            const int bt = fdiv(blk, g.d_nzb), zb = blk - bt * g.nzb;
+            double *o = meas + kMeasAcc * ((zb * g.cpr + jc) * g.n[3] + (bt << g.lbt) + lt);
+#pragma unroll
+            for (int i = 0; i < kMeasAcc; i++) o[i] = s[i];
+        }
     }
-    store_link<T, R>(wout, g, mu, idx, w);
 }
 
 // Dynamic LDS reserved per block; nonzero values cap the blocks resident per
@@ -172,8 +339,14 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
 #define GLACX_FLOW_LDS_BYTES 0
 #endif
 
+// Fusing the measurement needs each block's 32 sites per parity to be one
+// chunk of one run (a single time slice): run % 32 == 0, which holds for
+// every lattice with bz Ny Nx / 2 divisible by 32.
+bool flow_measure_fusable(const Geom &g) { return g.run % (kFlowSites / 2) == 0; }
+
 template <class T, int R>
-void launch_flow_stage(const void *win, void *wout, void *acc, const Geom &g, int stage, double eps, hipStream_t s)
+void launch_flow_stage(const void *win, void *wout, void *acc, const Geom &g, int stage, double eps, double *meas,
+                       hipStream_t s)
 {
     using V2 = typename vec2<T>::type;
     const unsigned blocks = (unsigned)((g.vh + kFlowSites / 2 - 1) / (kFlowSites / 2));
@@ -182,19 +355,32 @@ void launch_flow_stage(const void *win, void *wout, void *acc, const Geom &g, in
     T *a = (T *)acc;
     switch (stage) {
     case 0:
-        flow_stage_kernel<T, R, 0><<<blocks, kBlock, GLACX_FLOW_LDS_BYTES, s>>>(in, out, a, g, T(0.25 * eps), T(0));
+        if constexpr (sizeof(T) == 4) {
+            if (meas) {
+                flow_stage_kernel<T, R, 0, true>
+                    <<<blocks, kBlock, GLACX_FLOW_LDS_BYTES, s>>>(in, out, a, g, T(0.25 * eps), T(0), meas);
+                break;
+            }
+        }
+        flow_stage_kernel<T, R, 0, false>
+            <<<blocks, kBlock, GLACX_FLOW_LDS_BYTES, s>>>(in, out, a, g, T(0.25 * eps), T(0), nullptr);
         break;
     case 1:
-        flow_stage_kernel<T, R, 1><<<blocks, kBlock, GLACX_FLOW_LDS_BYTES, s>>>(in, out, a, g, T(8.0 / 9.0 * eps), T(17.0 / 36.0 * eps));
+        flow_stage_kernel<T, R, 1, false><<<blocks, kBlock, GLACX_FLOW_LDS_BYTES, s>>>(
+            in, out, a, g, T(8.0 / 9.0 * eps), T(17.0 / 36.0 * eps), nullptr);
         break;
     default:
-        flow_stage_kernel<T, R, 2><<<blocks, kBlock, GLACX_FLOW_LDS_BYTES, s>>>(in, out, a, g, T(0.75 * eps), T(1));
+        flow_stage_kernel<T, R, 2, false>
+            <<<blocks, kBlock, GLACX_FLOW_LDS_BYTES, s>>>(in, out, a, g, T(0.75 * eps), T(1), nullptr);
         break;
     }
 }
 
-template void launch_flow_stage<float, 3>(const void *, void *, void *, const Geom &, int, double, hipStream_t);
-template void launch_flow_stage<float, 2>(const void *, void *, void *, const Geom &, int, double, hipStream_t);
-template void launch_flow_stage<double, 3>(const void *, void *, void *, const Geom &, int, double, hipStream_t);
+template void launch_flow_stage<float, 3>(const void *, void *, void *, const Geom &, int, double, double *,
+                                         hipStream_t);
+template void launch_flow_stage<float, 2>(const void *, void *, void *, const Geom &, int, double, double *,
+                                         hipStream_t);
+template void launch_flow_stage<double, 3>(const void *, void *, void *, const Geom &, int, double, double *,
+                                          hipStream_t);
 
 } // namespace glacx

@@ -92,10 +92,6 @@ template <class T> __device__ __forceinline__ M3<T> lds_get(int slot, int lane)
     return m;
 }
 
-// Partials: kMeasAcc doubles per (chunk, t) slot and per time slice:
-// {plaq, E, q, w}; w (the Weinberg density) only in the SuperSampler variant.
-constexpr int kMeasAcc = 4;
-
 // SUPER adds GLAC's SuperSampler Weinberg operator
 // (GLAC/observables/supersampler.cpp:430-460): per site
 //   w = -sum_{nu=1..3} sum_{lambda != 0, nu} Re tr(G_{0 lambda} G_{nu lambda} G_{rho sigma}).

@@ -61,6 +61,11 @@ template <class T, int R> void launch_cold_start(void *f, const Geom &g, hipStre
 template <class T, int R>
 void launch_action_derivative(const void *f, double *stage, const Geom &g, int mu, hipStream_t s);
 
+// Clover measurement partials: kMeasAcc doubles {plaq, E, q, w} per
+// (chunk, t) slot, chunk c of time slice t at partials[kMeasAcc (c Nt + t)]
+// (w, the Weinberg density, only in the SuperSampler variant).
+constexpr int kMeasAcc = 4;
+
 // Clover measurement: per-block partial sums, then a one-block fixed-order
 // reduction into out = {plaq, E_raw, Q, Q(t)...}, or with `super` (GLAC's
 // SuperSampler) {plaq, E_raw, Q, W, Q(t)..., W(t)...}.  mode: 0 plain,
@@ -76,8 +81,13 @@ template <class T, int R> void launch_plaquette(const void *f, const Geom &g, do
 void launch_plaquette_finish(const double *partials, int nparts, const Geom &g, double *out, hipStream_t s);
 
 // One RK3 stage (0, 1, 2): win -> wout, accumulator read/written in acc.
+// With `meas` (stage 0 only, FP32 links, flow_measure_fusable(g)) the stage
+// also writes the clover-measurement partials of win into meas, in the layout
+// launch_measure_finish reads.
 template <class T, int R>
-void launch_flow_stage(const void *win, void *wout, void *acc, const Geom &g, int stage, double eps, hipStream_t s);
+void launch_flow_stage(const void *win, void *wout, void *acc, const Geom &g, int stage, double eps, double *meas,
+                       hipStream_t s);
+bool flow_measure_fusable(const Geom &g);
 
 // One checkerboard sub-sweep (mu, parity), in place; accepted hits are added
 // to *count.

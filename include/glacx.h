@@ -59,7 +59,8 @@ enum glacx_kernel {
     GLACX_K_PLAQUETTE = 3,
     GLACX_K_METROPOLIS_LEARNED = 4,
     GLACX_K_TRAIN = 5,
-    GLACX_K_COUNT = 6
+    GLACX_K_FLOW_STAGE_MEASURE = 6, /* stage 0 with the clover observables fused (FP32 flow with measurement) */
+    GLACX_K_COUNT = 7
 };
 
 /* Create a context for an Nx x Ny x Nz x Nt periodic lattice (all extents
