@@ -14,7 +14,9 @@ Also reported, each from its own timed region:
    sweep) and FP64 flow steps/s;
  - ``roofline``: the flow-stage kernel's algorithmic bytes per launch over its
    mean launch time (HIP events on the library's stream), against 8 TB/s, with
-   the HBM traffic per launch measured by rocprofv3 (profiles/);
+   the HBM traffic per launch measured by rocprofv3 (profiles/).  With the
+   measurement fused into stage 0 (FP32 links), that is RK3 stages 1 and 2;
+   ``stage0_measure`` reports the fused stage 0 the same way;
  - ``indep_configs_per_s``: effectively independent configurations per second,
    configs/s / (2 tau_int(Q^2)), for the learned proposal with FP32 links and
    for GLAC's plain RST proposal with FP64 links (``--indep-seconds`` each);
@@ -47,11 +49,14 @@ def alg_bytes(fp32):
     return 32 if fp32 else 64       # 8 reals per su(3) accumulator entry
 
 
-def flow_stage_bytes_per_launch(V, fp32, rows=3):
-    """Per RK3 step the three stage launches read 4V links and write 4V links
-    each, and move the su(3) accumulator 4V entries (write), 8V (read+write),
-    4V (read): 24 s_L + 16 s_Z per site per step, a third per launch."""
-    return (24 * link_bytes(fp32, rows) + 16 * alg_bytes(fp32)) * V / 3.0
+def flow_stage_bytes_per_launch(V, fp32, rows=3, stages=(0, 1, 2)):
+    """Each RK3 stage launch reads 4V links and writes 4V links, and moves the
+    su(3) accumulator: 4V entries written in stage 0, 4V read and written in
+    stage 1, 4V read in stage 2 (24 s_L + 16 s_Z per site per step).  Mean
+    over `stages`."""
+    acc = {0: 4, 1: 8, 2: 4}
+    per_site = [8 * link_bytes(fp32, rows) + acc[k] * alg_bytes(fp32) for k in stages]
+    return sum(per_site) * V / len(per_site)
 
 
 def metropolis_bytes_per_launch(V, fp32, rows=3):
@@ -226,9 +231,13 @@ def main():
         f"stage 0 + clover {fm_ms / max(fm_launches, 1):.4f} ms x {fm_launches}, "
         f"clover kernel {meas_ms / max(meas_launches, 1):.4f} ms x {meas_launches}")
 
-    stage_bytes = flow_stage_bytes_per_launch(V, True, rows)
+    # With the measurement fused into stage 0, the plain stage kernel is timed
+    # on stages 1 and 2 only.
+    plain_stages = (1, 2) if fm_launches else (0, 1, 2)
+    stage_bytes = flow_stage_bytes_per_launch(V, True, rows, plain_stages)
     stage_ms = f_ms / max(f_launches, 1)
     achieved = stage_bytes / (stage_ms * 1e-3) / 1e9
+    stage0_bytes = flow_stage_bytes_per_launch(V, True, rows, (0,))
     metro_bytes = metropolis_bytes_per_launch(V, True, rows)
     metro_ms = m_ms / max(m_launches, 1)
     metro_gbs = metro_bytes / (metro_ms * 1e-3) / 1e9
@@ -293,14 +302,19 @@ def main():
         "config": {"workload": "RK3 Wilson flow step + clover measurement", "lattice": list(n),
                    "beta": beta, "flow_eps": eps_flow, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(n, f"flow_stage_kernel<float, {rows}>"),
-                     "kernel": f"flow_stage_kernel<float, {rows}>",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(n, f"flow_stage_kernel<float, {rows}>"),
+                     "kernel": f"flow_stage_kernel<float, {rows}, {{{','.join(map(str, plain_stages))}}}, false>",
                      "alg_bytes_per_launch": stage_bytes, "mean_launch_ms": stage_ms},
         "metropolis": {"sweeps_per_s": sweeps_per_s, "acceptance": float(acc.mean()),
                        "subsweep_kernel_ms": metro_ms, "subsweep_kernel_gbs": metro_gbs,
                        "hbm_frac": metro_gbs / HBM_PEAK_GBS},
         "measure_kernel_ms": meas_ms / max(meas_launches, 1),
-        "stage0_measure_kernel_ms": fm_ms / max(fm_launches, 1) if fm_launches else None,
+        "stage0_measure": ({"kernel": f"flow_stage_kernel<float, {rows}, 0, true>", "mean_launch_ms": fm_ms / fm_launches,
+                            "alg_bytes_per_launch": stage0_bytes,
+                            "achieved_gbs": stage0_bytes / (fm_ms / fm_launches * 1e-3) / 1e9,
+                            "traffic": profiled_traffic(n, f"flow_stage_kernel<float, {rows}, 0, true>")}
+                           if fm_launches else None),
         "fp64_plain": fp64,
         "indep_configs_per_s": indep,
         "cpu_baseline": cpu,
