@@ -214,19 +214,30 @@ def main():
     L.copy(G.FLOW, G.MAIN)
     L.flow(args.warmup, eps_flow, field=G.FLOW, measure=True)
     L.synchronize()
-    L.timing(True)
-    L.timing_reset()
     t0 = time.perf_counter()
     plaq, e_raw, q, qt = L.flow(args.steps, eps_flow, field=G.FLOW, measure=True)
     L.synchronize()
     elapsed = time.perf_counter() - t0
+    # Per-kernel times: the same flow again from the same field, with a HIP
+    # event pair around every launch (kept out of the run above, whose wall
+    # time is the step rate).
+    L.copy(G.FLOW, G.MAIN)
+    L.flow(args.warmup, eps_flow, field=G.FLOW, measure=True)
+    L.synchronize()
+    L.timing(True)
+    L.timing_reset()
+    t1 = time.perf_counter()
+    L.flow(args.steps, eps_flow, field=G.FLOW, measure=True)
+    L.synchronize()
+    elapsed_events = time.perf_counter() - t1
     f_launches, f_ms = L.timing_read(G.K_FLOW_STAGE)
     fm_launches, fm_ms = L.timing_read(G.K_FLOW_STAGE_MEASURE)
     meas_launches, meas_ms = L.timing_read(G.K_MEASURE)
     L.timing(False)
     L.close()
     steps_per_s = args.steps / elapsed
-    log(f"[bench] flow {args.precision}: {args.steps} steps in {elapsed:.3f} s = {steps_per_s:.1f} steps/s; "
+    log(f"[bench] flow {args.precision}: {args.steps} steps in {elapsed:.3f} s = {steps_per_s:.1f} steps/s "
+        f"({args.steps / elapsed_events:.1f} with per-kernel events); "
         f"stage kernel {f_ms / max(f_launches, 1):.4f} ms x {f_launches}, "
         f"stage 0 + clover {fm_ms / max(fm_launches, 1):.4f} ms x {fm_launches}, "
         f"clover kernel {meas_ms / max(meas_launches, 1):.4f} ms x {meas_launches}")
@@ -310,6 +321,7 @@ def main():
                        "subsweep_kernel_ms": metro_ms, "subsweep_kernel_gbs": metro_gbs,
                        "hbm_frac": metro_gbs / HBM_PEAK_GBS},
         "measure_kernel_ms": meas_ms / max(meas_launches, 1),
+        "flow_steps_per_s_with_kernel_events": args.steps / elapsed_events,
         "stage0_measure": ({"kernel": f"flow_stage_kernel<float, {rows}, 0, true>", "mean_launch_ms": fm_ms / fm_launches,
                             "alg_bytes_per_launch": stage0_bytes,
                             "achieved_gbs": stage0_bytes / (fm_ms / fm_launches * 1e-3) / 1e9,

@@ -238,13 +238,22 @@ template <class T> __device__ __forceinline__ M3<T> exp_su3(const su3alg<T> &y)
     const double c0 = a0 * a1 * a2 + 2.0 * (tr * h02r + ti * h02i) - a0 * n12 - a1 * n02 - a2 * n01;
 
     const M3<T> Y = expand(y);
+    // Series order: N terms leave a remainder below ||H||^(N+1) / (N+1)!,
+    // ||H||_F^2 = 2 c1, under FP64 rounding for c1 <= 2e-6 (N = 5), 2e-4 (7),
+    // 0.005 (9), 0.5 (20); the closed form beyond.  The order is the largest
+    // any series lane of the wavefront needs, so the series is one path.
     ExpCoeffs f;
-    if (c1 <= 0.005)
-        f = exp_coefficients_series<12>(c0, c1);
-    else if (c1 <= 0.5)
-        f = exp_coefficients_series<20>(c0, c1);
-    else
+    const bool series = c1 <= 0.5;
+    if (!series)
         f = exp_coefficients(c0, c1);
+    else if (__any(c1 > 0.005))
+        f = exp_coefficients_series<20>(c0, c1);
+    else if (__any(c1 > 2e-4))
+        f = exp_coefficients_series<9>(c0, c1);
+    else if (__any(c1 > 2e-6))
+        f = exp_coefficients_series<7>(c0, c1);
+    else
+        f = exp_coefficients_series<5>(c0, c1);
     // H = -iY, H^2 = -Y^2:  X = f0 1 + f1 (-iY) + f2 (-Y^2)
     return exp_assemble<T>(Y, T(f.f[0][0]), T(f.f[0][1]), T(f.f[1][0]), T(f.f[1][1]), T(f.f[2][0]), T(f.f[2][1]));
 }

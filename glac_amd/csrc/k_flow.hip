@@ -298,7 +298,7 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
             // (|U U^dag - 1| ~ 1e-4 after 1000 steps at 24^3 x 48), and that drift,
             // not the FP32 measurement, dominates the error of E(t) at large t.
             M3<double> d = convert<double>(w);
-            reunitarize<R>(d);
+            reunitarize<R, true>(d);
             w = convert<T>(d);
         }
         store_link<T, R>(wout, g, mu, idx, w);
