@@ -147,7 +147,7 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--dims", default="32,32,32,32")
     ap.add_argument("--sweeps", type=int, default=100, help="timed Metropolis sweeps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -159,10 +159,19 @@ def main():
                          "measurement (learned FP32 vs plain FP64); 0 skips it")
     ap.add_argument("--only-indep", action="store_true",
                     help="run only the independent-configs/s measurement and print its JSON")
+    ap.add_argument("--calibration", action="store_true",
+                    help="only run glac_amd/calibrate (FP32/FP64 FMA issue rate, L1 bandwidth) and print its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy kernel (for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+
+    if args.calibration:
+        import subprocess
+        exe = os.path.join(ROOT, "glac_amd", "calibrate")
+        if not os.path.exists(exe):
+            sys.exit(f"{exe} is not built; run __graft_entry__.build()")
+        sys.exit(subprocess.run([exe]).returncode)
 
     if args.copy_calibration:
         import torch
