@@ -312,25 +312,29 @@ __global__ __launch_bounds__(kBlock) void to_bf16_kernel(const float *__restrict
 
 // ---- sampling with the learned proposal ----------------------------------------
 template <class T, int R>
-__global__ __launch_bounds__(kBlock) void metropolis_learned_kernel(typename vec2<T>::type *__restrict__ f, Geom g,
+__global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 3 : 1) void metropolis_learned_kernel(typename vec2<T>::type *__restrict__ f, Geom g,
                                                                     int mu, int parity, MetroParams prm,
                                                                     const unsigned short *__restrict__ wbf16,
                                                                     const float *__restrict__ mean_std,
                                                                     unsigned long long *__restrict__ count)
 {
-    __shared__ float P[kNParam];
-    for (int k = threadIdx.x; k < kNParam; k += kBlock) P[k] = bf16_to_float(wbf16[k]);
+    __shared__ __align__(16) MlpImage im;
+    __shared__ __align__(16) float obuf[kBlock / 64][64 * kMlpOut];
+    mlp_image_fill(im, wbf16, threadIdx.x, kBlock);
     __syncthreads();
-    const int h = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const int h_raw = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const bool valid = h_raw < g.vh;
+    const int h = valid ? h_raw : g.vh - 1;  // idle lanes mirror a real link for the network GEMMs, store nothing
     double nacc[1] = {0.0};
-    if (h < g.vh) {
+    {
         const int idx = parity_index(g, parity, h);
         const Coord x = decode(g, idx);
         const M3<T> a = staple<T, R>(f, g, x, mu);
         M3<T> u = load_link<T, R>(f, g, mu, idx);
         float fe[6], o[36], L[36];
         staple_features(a, prm.beta, fe);
-        const bool learned = mlp_forward(P, mean_std, mean_std + 6, fe, o);
+        const bool learned =
+            mlp_forward_mfma(im, mean_std, mean_std + 6, fe, obuf[threadIdx.x >> 6], o);  // whole wavefront
         cholesky_from_output(o, L);
         Stream rs(prm.seed, (uint32_t)lex_of(g, x), (uint32_t)mu, STREAM_LEARNED, prm.sweep);
         const double bf = prm.beta / 3.0;
@@ -365,13 +369,15 @@ __global__ __launch_bounds__(kBlock) void metropolis_learned_kernel(typename vec
                 accepted++;
             }
         }
-        if (prm.reunit) {
-            M3<double> d = convert<double>(u);
-            reunitarize(d);
-            u = convert<T>(d);
+        if (valid) {
+            if (prm.reunit) {
+                M3<double> d = convert<double>(u);
+                reunitarize(d);
+                u = convert<T>(d);
+            }
+            store_link<T, R>(f, g, mu, idx, u);
+            nacc[0] = (double)accepted;
         }
-        store_link<T, R>(f, g, mu, idx, u);
-        nacc[0] = (double)accepted;
     }
     block_sum<1>(nacc);
     if (threadIdx.x == 0) atomicAdd(count, (unsigned long long)nacc[0]);

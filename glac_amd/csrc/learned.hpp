@@ -99,6 +99,151 @@ __device__ __forceinline__ bool mlp_forward(const float *P, const float *mean, c
     return ok;
 }
 
+// ---- the same network on the matrix cores ------------------------------------
+// A wavefront evaluates the network for its 64 links as three bf16 GEMMs on
+// v_mfma_f32_16x16x16_bf16 (FP32 accumulation): weights are the A operand
+// (rows = units), activations the B operand (columns = links), 16 links per
+// tile.  Lane l of a 16x16 tile holds A[l % 16][4 (l / 16) + 0..3],
+// B[4 (l / 16) + 0..3][l % 16] and D[4 (l / 16) + 0..3][l % 16], so the D
+// tile of one layer is, element for element, the B fragment of the next:
+// only the input features (one link per lane, moved with lane shuffles) and
+// the outputs (back to one link per lane, through LDS) change layout.  Layer
+// inputs are rounded to bf16 (round to nearest even), as the weights are.
+using bf16x4 = short __attribute__((ext_vector_type(4)));
+using f32x4 = float __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned pack_bf16(float a, float b)
+{
+    return (unsigned)__builtin_bit_cast(unsigned short, (__bf16)a) |
+           ((unsigned)__builtin_bit_cast(unsigned short, (__bf16)b) << 16);
+}
+__device__ __forceinline__ bf16x4 frag_bf16(unsigned lo, unsigned hi)
+{
+    using U2 = unsigned __attribute__((ext_vector_type(2)));
+    U2 v;
+    v.x = lo;
+    v.y = hi;
+    return __builtin_bit_cast(bf16x4, v);
+}
+__device__ __forceinline__ bf16x4 lrelu_bf16(const f32x4 &d)
+{
+    return frag_bf16(pack_bf16(lrelu(d[0]), lrelu(d[1])), pack_bf16(lrelu(d[2]), lrelu(d[3])));
+}
+// The network's parameters as the GEMMs read them (bf16 bits, zero padded so
+// every fragment is one aligned 8-byte LDS read): W1 with rows of 8, W3 and
+// b3 with 48 rows.
+struct MlpImage {
+    unsigned short w1[kMlpH1][8];
+    unsigned short b1[kMlpH1];
+    unsigned short w2[kMlpH2][kMlpH1];
+    unsigned short b2[kMlpH2];
+    unsigned short w3[48][kMlpH2];
+    unsigned short b3[48];
+};
+
+// Fill the image from the packed bf16 parameters (all threads of a block).
+__device__ __forceinline__ void mlp_image_fill(MlpImage &im, const unsigned short *__restrict__ wbf16, int tid,
+                                               int nthreads)
+{
+    unsigned short *dst = &im.w1[0][0];
+    constexpr int n = (int)(sizeof(MlpImage) / sizeof(unsigned short));
+    for (int k = tid; k < n; k += nthreads) {
+        int src = -1;
+        if (k < kMlpH1 * 8) {
+            const int r = k >> 3, c = k & 7;
+            src = c < kMlpIn ? kOffW1 + r * kMlpIn + c : -1;
+        } else if (k < kMlpH1 * 9) {
+            src = kOffB1 + (k - kMlpH1 * 8);
+        } else if (k < kMlpH1 * 9 + kMlpH2 * kMlpH1) {
+            src = kOffW2 + (k - kMlpH1 * 9);
+        } else if (k < kMlpH1 * 9 + kMlpH2 * (kMlpH1 + 1)) {
+            src = kOffB2 + (k - kMlpH1 * 9 - kMlpH2 * kMlpH1);
+        } else if (k < kMlpH1 * 9 + kMlpH2 * (kMlpH1 + 1) + 48 * kMlpH2) {
+            const int j = k - (kMlpH1 * 9 + kMlpH2 * (kMlpH1 + 1));
+            src = (j >> 5) < kMlpOut ? kOffW3 + j : -1;
+        } else {
+            const int j = k - (kMlpH1 * 9 + kMlpH2 * (kMlpH1 + 1) + 48 * kMlpH2);
+            src = j < kMlpOut ? kOffB3 + j : -1;
+        }
+        dst[k] = src >= 0 ? wbf16[src] : (unsigned short)0;
+    }
+}
+
+__device__ __forceinline__ bf16x4 lds_frag(const unsigned short *p)
+{
+    return __builtin_bit_cast(bf16x4, *reinterpret_cast<const unsigned long long *>(p));
+}
+__device__ __forceinline__ f32x4 lds_bias(const unsigned short *p)
+{
+    const unsigned long long v = *reinterpret_cast<const unsigned long long *>(p);
+    f32x4 c;
+#pragma unroll
+    for (int e = 0; e < 4; e++) c[e] = __uint_as_float((unsigned)((v >> (16 * e)) & 0xFFFFu) << 16);
+    return c;
+}
+
+// Network outputs o[36] of this lane's link; every lane of the wavefront must
+// call it (the GEMMs span the wavefront).  im: the parameter image (LDS),
+// obuf: this wavefront's 64 x 36 floats of LDS (16-byte aligned).  Returns
+// false when a standardised feature is out of range.
+__device__ __forceinline__ bool mlp_forward_mfma(const MlpImage &im, const float *mean, const float *stdv,
+                                                 const float f[6], float *obuf, float o[36])
+{
+    const int lane = threadIdx.x & 63, row = lane & 15, grp = lane >> 4;
+    float x[kMlpIn];
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < kMlpIn; k++) {
+        x[k] = (f[k] - mean[k]) / stdv[k];
+        ok = ok && fabsf(x[k]) <= kOutOfRange;
+    }
+    const unsigned p0 = pack_bf16(x[0], x[1]), p1 = pack_bf16(x[2], x[3]), p2 = pack_bf16(x[4], x[5]);
+#pragma unroll 1
+    for (int nt = 0; nt < 4; nt++) {
+        // parameters are re-read from LDS per tile rather than held in registers
+        asm volatile("" ::: "memory");
+        // features of link 16 nt + l % 16, k = 4 grp + 0..3 (k < 6)
+        const int src = 16 * nt + row;
+        const unsigned q0 = (unsigned)__shfl((int)p0, src, 64), q1 = (unsigned)__shfl((int)p1, src, 64),
+                       q2 = (unsigned)__shfl((int)p2, src, 64);
+        const bf16x4 b = frag_bf16(grp == 0 ? q0 : (grp == 1 ? q2 : 0u), grp == 0 ? q1 : 0u);
+        bf16x4 a1[2], a2[2];
+#pragma unroll
+        for (int m = 0; m < 2; m++) {
+            bf16x4 w = lds_frag(&im.w1[16 * m + row][4 * (grp & 1)]);
+            if (grp >= 2) w = frag_bf16(0u, 0u);
+            a1[m] = lrelu_bf16(__builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w, b, lds_bias(&im.b1[16 * m + 4 * grp]), 0, 0, 0));
+        }
+#pragma unroll
+        for (int m = 0; m < 2; m++) {
+            f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds_frag(&im.w2[16 * m + row][4 * grp]), a1[0],
+                                                                lds_bias(&im.b2[16 * m + 4 * grp]), 0, 0, 0);
+            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds_frag(&im.w2[16 * m + row][16 + 4 * grp]), a1[1], d, 0, 0, 0);
+            a2[m] = lrelu_bf16(d);
+        }
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds_frag(&im.w3[16 * m + row][4 * grp]), a2[0],
+                                                                lds_bias(&im.b3[16 * m + 4 * grp]), 0, 0, 0);
+            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds_frag(&im.w3[16 * m + row][16 + 4 * grp]), a2[1], d, 0, 0, 0);
+            const int r0 = 16 * m + 4 * grp;
+            if (r0 < kMlpOut) *reinterpret_cast<f32x4 *>(obuf + (16 * nt + row) * kMlpOut + r0) = d;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the wavefront's LDS stores have landed
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int i = 0; i < kMlpOut; i += 4) {
+        const f32x4 v = *reinterpret_cast<const f32x4 *>(obuf + lane * kMlpOut + i);
+        o[i] = v[0];
+        o[i + 1] = v[1];
+        o[i + 2] = v[2];
+        o[i + 3] = v[3];
+    }
+    return ok;
+}
+
 // packed lower triangle, row a holds L[a][0..a] at a(a+1)/2
 __device__ __forceinline__ void cholesky_from_output(const float o[36], float L[36])
 {

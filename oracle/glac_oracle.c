@@ -942,20 +942,34 @@ void or_staple_features(const double *A, double beta, double f[6])
 
 static inline double lrelu(double v) { return v > 0.0 ? v : 0.01 * v; }
 
+/* v as FP32, then to bf16 with round-to-nearest-even (finite values) */
+static double bf16_round(double v)
+{
+    float x = (float)v;
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+    memcpy(&x, &u, 4);
+    return (double)x;
+}
+
+/* The device evaluates the network as bf16 GEMMs with FP32 accumulation:
+ * each layer's input (standardised features, hidden activations) is rounded
+ * to bf16, as the weights are (the caller passes bf16-valued parameters). */
 void or_mlp_forward(const double *P, const double *mean, const double *stdv, const double f[6], double o[36])
 {
     const double *W1 = P, *b1 = W1 + 32 * 6, *W2 = b1 + 32, *b2 = W2 + 32 * 32, *W3 = b2 + 32, *b3 = W3 + 36 * 32;
     double x[6], h1[32], h2[32];
-    for (int k = 0; k < 6; k++) x[k] = (f[k] - mean[k]) / stdv[k];
+    for (int k = 0; k < 6; k++) x[k] = bf16_round((f[k] - mean[k]) / stdv[k]);
     for (int i = 0; i < 32; i++) {
         double s = b1[i];
         for (int k = 0; k < 6; k++) s += W1[i * 6 + k] * x[k];
-        h1[i] = lrelu(s);
+        h1[i] = bf16_round(lrelu(s));
     }
     for (int i = 0; i < 32; i++) {
         double s = b2[i];
         for (int k = 0; k < 32; k++) s += W2[i * 32 + k] * h1[k];
-        h2[i] = lrelu(s);
+        h2[i] = bf16_round(lrelu(s));
     }
     for (int i = 0; i < 36; i++) {
         double s = b3[i];

@@ -117,3 +117,29 @@ def test_learned_sweep_reproduces_recorded_plaquette():
         pl.append(O.measure(U, n)[0])
     assert abs(np.mean(pl) - 0.59586) < 0.008
     assert 0.05 < np.mean(acc) < 0.95
+
+
+def test_mlp_rounds_layer_inputs_to_bf16():
+    """An identity network (W1, W2, W3 = unit rows, biases 0) passes input 0
+    through three bf16 roundings: 1 + 3/512 lies between the bf16 values 1
+    and 1 + 1/128 and rounds to the nearer, 1 + 1/128 (truncation would give
+    1, no rounding 1 + 3/512); a negative input goes through the leaky ReLU
+    (slope 0.01) twice before each rounding."""
+    P = np.zeros(2468)
+    W1, b1 = 0, 192
+    W2, b2 = 224, 1248
+    W3, b3 = 1280, 2432
+    for i in range(6):
+        P[W1 + i * 6 + i] = 1.0
+    for i in range(32):
+        P[W2 + i * 32 + i] = 1.0
+        P[W3 + i * 32 + i] = 1.0
+    f = np.array([1 + 3 / 512, -1.0, 0.0, 0.0, 0.0, 0.0])
+    o = O.mlp_forward(P, np.zeros(6), np.ones(6), f)
+    assert o[0] == 1 + 1 / 128
+    # -1 -> lrelu -0.01 -> bf16 -> lrelu -> bf16: compare with the two-step rounding
+    def bf16(v):
+        u = np.array([v], dtype=np.float32).view(np.uint32).astype(np.uint64)
+        return float(((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32).view(np.float32)[0])
+    assert o[1] == bf16(0.01 * bf16(-0.01))
+    assert o[1] != 0.01 * -0.01
