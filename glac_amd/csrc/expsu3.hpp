@@ -151,8 +151,7 @@ template <int N> __device__ __forceinline__ ExpCoeffs exp_coefficients_series(do
 }
 
 // FP32 variant of the series for the learned proposal's step matrix with FP32
-// links: 16 terms bound the truncation by ||H||^16 / 16! < 3e-9 for
-// 2 c1 = ||H||_F^2 <= 4, below FP32 rounding.
+// links (N terms: truncation below ||H||^(N+1) / (N+1)!).
 template <int N> __device__ __forceinline__ void exp_coefficients_series_f32(float c0, float c1, float f[3][2])
 {
     float a = 1.f, b = 0.f, d = 0.f;
@@ -213,11 +212,18 @@ template <class T> __device__ __forceinline__ M3<T> exp_su3_proposal(const su3al
         const float c1 = 0.5f * (a0 * a0 + a1 * a1 + a2 * a2) + n01 + n02 + n12;
         const float tr = h01r * h12r - h01i * h12i, ti = h01r * h12i + h01i * h12r;
         const float c0 = a0 * a1 * a2 + 2.f * (tr * h02r + ti * h02i) - a0 * n12 - a1 * n02 - a2 * n01;
+        // N terms leave less than ||H||^(N+1) / (N+1)!, ||H||_F^2 = 2 c1, below
+        // FP32 rounding (6e-8) for c1 <= 0.125 (N = 8), 0.5 (11), 2 (16); the
+        // wavefront takes the largest order any of its series lanes needs.
         float f[3][2];
-        if (c1 <= 2.0f)
-            exp_coefficients_series_f32<16>(c0, c1, f);
-        else
+        if (c1 > 2.0f)
             exp_coefficients_f32(c0, c1, f);
+        else if (__any(c1 > 0.5f))
+            exp_coefficients_series_f32<16>(c0, c1, f);
+        else if (__any(c1 > 0.125f))
+            exp_coefficients_series_f32<11>(c0, c1, f);
+        else
+            exp_coefficients_series_f32<8>(c0, c1, f);
         return exp_assemble<float>(expand(y), f[0][0], f[0][1], f[1][0], f[1][1], f[2][0], f[2][1]);
     }
 }
