@@ -207,10 +207,12 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
             const int ipmmn = index_of(g, step(g, xpm, nu, -1));
             if constexpr (MEAS) {
                 // the same staple terms, kept apart for the clover leaves
-                const M3<T> p = su3_product<true>(load_rows01<T, R>(vnu, ipm), load_link<T, R>(vmu, ipn));
+                M3<T> p, up;  // p: first two rows of U_nu(x+mu) U_mu(x+nu)^dag
+                mm<false, true, false, 2>(p, load_rows01<T, R>(vnu, ipm), load_link<T, R>(vmu, ipn));
+                mm<false, true, false, 2>(up, p, lds_get(sh, nu, lane));
+                rebuild_third_row(up);
                 const M3<T> q = su3_product<false>(load_rows01<T, R>(vmu, imn), load_link<T, R>(vnu, ipmmn));
-                M3<T> up, low;
-                mm<false, true, false>(up, p, lds_get(sh, nu, lane));
+                M3<T> low;
                 mm<true, false, false>(low, q, lds_get(sh, 4 + nu, lane));
                 add_to(a, up);
                 add_to(a, low);
@@ -250,10 +252,14 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
                 // from operands of which only those are needed, and rebuild
                 // the product's third row once (36 multiply-adds fewer per
                 // product than rebuilding both operands).
-                M3<T> p;  // U_nu(x+mu) U_mu(x+nu)^dag
+                M3<T> p;  // U_nu(x+mu) U_mu(x+nu)^dag, first two rows
                 mm<false, true, false, 2>(p, load_rows01<T, R>(vnu, ipm), load_link<T, R>(vmu, ipn));
-                rebuild_third_row(p);
-                mm<false, true, true>(a, p, lds_get(sh, nu, lane));
+                // the upper term p U_nu(x)^dag is SU(3) too: its first two rows
+                // need only p's, and its third is rebuilt from them
+                M3<T> t;
+                mm<false, true, false, 2>(t, p, lds_get(sh, nu, lane));
+                rebuild_third_row(t);
+                add_to(a, t);
                 M3<T> q;  // U_mu(x-nu) U_nu(x+mu-nu), whose dagger the lower staple uses
                 mm<false, false, false, 2>(q, load_rows01<T, R>(vmu, imn), load_link<T, R>(vnu, ipmmn));
                 rebuild_third_row(q);
