@@ -170,7 +170,7 @@ template <class T> __device__ __forceinline__ FlowMeasLds<T> &flow_meas_lds()
 }
 
 template <class T, int R, int STAGE, bool MEAS>
-__global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<T>::type *__restrict__ win,
+__global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : 2) void flow_stage_kernel(const typename vec2<T>::type *__restrict__ win,
                                                             typename vec2<T>::type *__restrict__ wout,
                                                             T *__restrict__ acc, Geom g, T cz, T ca,
                                                             double *__restrict__ meas)
@@ -186,8 +186,15 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
     const int idx = parity_index(g, lane >> 5, h);
     const Coord x = decode(g, idx);
     const DirView<T> vmu = dir_view<T>(win, g, mu);
+    // the site's neighbours x + mu and x - mu, shared with the other
+    // wavefronts, whose staples step the same way in direction mu
+    __shared__ int nbr[2][4][kFlowSites];
+    const Coord xpm = step(g, x, mu, +1);
+    const int ipm = index_of(g, xpm), imm = index_of(g, step(g, x, mu, -1));
+    nbr[0][mu][lane] = ipm;
+    nbr[1][mu][lane] = imm;
     lds_put(sh, mu, lane, load_link<T, R>(vmu, idx));
-    lds_put(sh, 4 + mu, lane, load_link<T, R>(vmu, index_of(g, step(g, x, mu, -1))));
+    lds_put(sh, 4 + mu, lane, load_link<T, R>(vmu, imm));
     T plaq = T(0);
     __syncthreads();
     // staple A_mu(x) = sum_{nu != mu} [ U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
@@ -195,15 +202,12 @@ __global__ __launch_bounds__(kBlock) void flow_stage_kernel(const typename vec2<
     // (GLAC/actions/wilsongaugeaction.cpp:29-53)
     M3<T> a = zero3<T>();
     {
-        const Coord xpm = step(g, x, mu, +1);
-        const int ipm = index_of(g, xpm);
 #pragma unroll 1
         for (int k = 0; k < 3; k++) {
             const int e = 3 * mu + k;
             const int nu = MEAS ? (int)((kPhaseNu >> (4 * e)) & 0xF) : (mu + k + 1) & 3;
             const DirView<T> vnu = dir_view<T>(win, g, nu);
-            const int ipn = index_of(g, step(g, x, nu, +1));
-            const int imn = index_of(g, step(g, x, nu, -1));
+            const int ipn = nbr[0][nu][lane], imn = nbr[1][nu][lane];
             const int ipmmn = index_of(g, step(g, xpm, nu, -1));
             if constexpr (MEAS) {
                 // the same staple terms, kept apart for the clover leaves
