@@ -111,7 +111,7 @@ template <class T> __device__ __forceinline__ M3<T> lds_get(int slot, int lane)
 enum MeasMode { MEAS_PLAIN = 0, MEAS_SUPER = 1, MEAS_DENSITY = 2 };
 
 template <class T, int R, int MODE>
-__global__ __launch_bounds__(kMeasSites *kMeasWaves) void measure_kernel(const typename vec2<T>::type *__restrict__ f,
+__global__ __launch_bounds__(kMeasSites * kMeasWaves, sizeof(T) == 4 ? 3 : 2) void measure_kernel(const typename vec2<T>::type *__restrict__ f,
                                                                          Geom g, int cps, double *__restrict__ partials,
                                                                          double *__restrict__ density)
 {

@@ -24,7 +24,7 @@
 namespace glacx {
 
 template <class T, int R>
-__global__ __launch_bounds__(kBlock) void metropolis_kernel(typename vec2<T>::type *__restrict__ f, Geom g, int mu,
+__global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 2 : 1) void metropolis_kernel(typename vec2<T>::type *__restrict__ f, Geom g, int mu,
                                                             int parity, MetroParams prm,
                                                             unsigned long long *__restrict__ count)
 {
