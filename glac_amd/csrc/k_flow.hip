@@ -211,13 +211,23 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : 2) void f
             const int ipmmn = index_of(g, step(g, xpm, nu, -1));
             if constexpr (MEAS) {
                 // the same staple terms, kept apart for the clover leaves
-                M3<T> p, up;  // p: first two rows of U_nu(x+mu) U_mu(x+nu)^dag
-                mm<false, true, false, 2>(p, load_rows01<T, R>(vnu, ipm), load_link<T, R>(vmu, ipn));
-                mm<false, true, false, 2>(up, p, lds_get(sh, nu, lane));
-                rebuild_third_row(up);
-                const M3<T> q = su3_product<false>(load_rows01<T, R>(vmu, imn), load_link<T, R>(vnu, ipmmn));
-                M3<T> low;
-                mm<true, false, false>(low, q, lds_get(sh, 4 + nu, lane));
+                M3<T> up, low;
+                if constexpr (R == 2) {
+                    // SU(3) links (checked at upload): products from two rows,
+                    // third rows rebuilt
+                    M3<T> p;  // first two rows of U_nu(x+mu) U_mu(x+nu)^dag
+                    mm<false, true, false, 2>(p, load_rows01<T, R>(vnu, ipm), load_link<T, R>(vmu, ipn));
+                    mm<false, true, false, 2>(up, p, lds_get(sh, nu, lane));
+                    rebuild_third_row(up);
+                    const M3<T> q = su3_product<false>(load_rows01<T, R>(vmu, imn), load_link<T, R>(vnu, ipmmn));
+                    mm<true, false, false>(low, q, lds_get(sh, 4 + nu, lane));
+                } else {
+                    // three stored rows may hold GLAC's det = -1 links: full products
+                    mm<false, true, false>(up, mul_ad(load_link<T, R>(vnu, ipm), load_link<T, R>(vmu, ipn)),
+                                           lds_get(sh, nu, lane));
+                    mm<false, false, false>(low, mul_dd(load_link<T, R>(vnu, ipmmn), load_link<T, R>(vmu, imn)),
+                                            lds_get(sh, 4 + nu, lane));
+                }
                 add_to(a, up);
                 add_to(a, low);
                 const M3<T> u = lds_get(sh, mu, lane);
@@ -236,8 +246,14 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : 2) void f
                     // L3 = (U_d2(y) U_d1(x-d1))^dag (U_d1(y) U_d2(x-d2)),  y = x - d1 - d2
                     const int iy = index_of(g, step(g, step(g, x, d1, -1), d2, -1));
                     const DirView<T> v1 = first ? vmu : vnu, v2 = first ? vnu : vmu;
-                    const M3<T> m1 = su3_product<false>(load_rows01<T, R>(v1, iy), lds_get(sh, 4 + d2, lane));
-                    const M3<T> m2 = su3_product<false>(load_rows01<T, R>(v2, iy), lds_get(sh, 4 + d1, lane));
+                    M3<T> m1, m2;
+                    if constexpr (R == 2) {
+                        m1 = su3_product<false>(load_rows01<T, R>(v1, iy), lds_get(sh, 4 + d2, lane));
+                        m2 = su3_product<false>(load_rows01<T, R>(v2, iy), lds_get(sh, 4 + d1, lane));
+                    } else {
+                        m1 = mul(load_link<T, R>(v1, iy), lds_get(sh, 4 + d2, lane));
+                        m2 = mul(load_link<T, R>(v2, iy), lds_get(sh, 4 + d1, lane));
+                    }
                     T f3[8];
                     ta2_product<true>(m2, m1, f3);
 #pragma unroll
@@ -303,12 +319,18 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : 2) void f
         M3<T> w = zero3<T>();
         mm<false, false, false, R>(w, exp_su3(y), u);  // two rows suffice when two are stored
         if constexpr (sizeof(T) == 4) {
-            // FP32 links: project back onto SU(3) in FP64 every stage.  Without
-            // it the rounding of exp(Y) W drifts the links off the group
+            // FP32 links: project back onto the group in FP64 every stage.
+            // Without it the rounding of exp(Y) W drifts the links off it
             // (|U U^dag - 1| ~ 1e-4 after 1000 steps at 24^3 x 48), and that drift,
             // not the FP32 measurement, dominates the error of E(t) at large t.
+            // Three stored rows are orthonormalised row by row, which keeps
+            // det U (GLAC's det = -1 starts stay det = -1); two stored rows are
+            // SU(3) and the third is rebuilt on load.
             M3<double> d = convert<double>(w);
-            reunitarize<R, true>(d);
+            if constexpr (R == 2)
+                reunitarize<2, true>(d);
+            else
+                orthonormalize_rows<true>(d);
             w = convert<T>(d);
         }
         store_link<T, R>(wout, g, mu, idx, w);

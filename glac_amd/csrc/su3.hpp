@@ -292,6 +292,40 @@ __device__ __forceinline__ double rsqrt_refined(double x)
     return y * fma(-0.5 * x * y, y, 1.5);
 }
 
+// Row Gram-Schmidt of all three rows in FP64: the nearest unitary matrix
+// in the row-by-row sense, with det U's phase kept (no SU(3) assumption).
+template <bool FAST_NORM = false> __device__ __forceinline__ void orthonormalize_rows(M3<double> &m)
+{
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+#pragma unroll
+        for (int q = 0; q < r; q++) {
+            // row r -= <row q, row r> row q  (<a, b> = sum conj(a) b)
+            double pr = 0.0, pi = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                pr += m.e[q][j].r * m.e[r][j].r + m.e[q][j].i * m.e[r][j].i;
+                pi += m.e[q][j].r * m.e[r][j].i - m.e[q][j].i * m.e[r][j].r;
+            }
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const double ar = m.e[q][j].r, ai = m.e[q][j].i;
+                m.e[r][j].r -= pr * ar - pi * ai;
+                m.e[r][j].i -= pr * ai + pi * ar;
+            }
+        }
+        double n = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) n += m.e[r][j].r * m.e[r][j].r + m.e[r][j].i * m.e[r][j].i;
+        n = FAST_NORM ? rsqrt_refined(n) : 1.0 / sqrt(n);
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            m.e[r][j].r *= n;
+            m.e[r][j].i *= n;
+        }
+    }
+}
+
 // Row Gram-Schmidt in FP64; row 2 = conj(row0 x row1) so det = +1.
 // FAST_NORM: the norms through rsqrt_refined (links stored in FP32).
 template <int ROWS = 3, bool FAST_NORM = false> __device__ __forceinline__ void reunitarize(M3<double> &m)
