@@ -5,8 +5,8 @@
 // is loaded.  For a link with staple A:
 //   features f(A) = (tr AA^dag, tr (AA^dag)^2, |det A|^2, Re det A, Im det A, beta),
 //       all invariant under A -> g A h^dagger, standardised with (mean, std);
-//   MLP 6 -> 32 -> 32 -> 36 (leaky ReLU, slope 0.01), weights in BF16,
-//       FP32 accumulation;
+//   MLP 6 -> 32 -> 32 -> 36 (leaky ReLU, slope 0.01), weights and layer
+//       inputs in bf16, FP32 accumulation (matrix cores, mlp_forward_mfma);
 //   output o -> lower-triangular L: L_aa = exp(o_a), L_ab = o[8 + a(a-1)/2 + b];
 //   per hit w = L z with z symmetric (z_a = (c1 + c2) sqrt 6, c uniform on the
 //       2^16 midpoints of [-1/2, 1/2), two per 32-bit output), X = exp(i sum_a w_a lambda_a / 2),
@@ -61,45 +61,7 @@ template <class T> __device__ __forceinline__ void staple_features(const M3<T> &
     f[5] = (float)beta;
 }
 
-// Network forward pass with parameters P (FP32 values, e.g. in LDS).
-// Returns false when a standardised feature is out of range.
-__device__ __forceinline__ bool mlp_forward(const float *P, const float *mean, const float *stdv, const float f[6],
-                                            float o[36])
-{
-    float x[kMlpIn];
-    bool ok = true;
-#pragma unroll
-    for (int k = 0; k < kMlpIn; k++) {
-        x[k] = (f[k] - mean[k]) / stdv[k];
-        ok = ok && fabsf(x[k]) <= kOutOfRange;
-    }
-    float h1[kMlpH1];
-#pragma unroll
-    for (int i = 0; i < kMlpH1; i++) {
-        float s = P[kOffB1 + i];
-#pragma unroll
-        for (int k = 0; k < kMlpIn; k++) s = fmaf(P[kOffW1 + i * kMlpIn + k], x[k], s);
-        h1[i] = lrelu(s);
-    }
-    float h2[kMlpH2];
-#pragma unroll
-    for (int i = 0; i < kMlpH2; i++) {
-        float s = P[kOffB2 + i];
-#pragma unroll
-        for (int k = 0; k < kMlpH1; k++) s = fmaf(P[kOffW2 + i * kMlpH1 + k], h1[k], s);
-        h2[i] = lrelu(s);
-    }
-#pragma unroll
-    for (int i = 0; i < kMlpOut; i++) {
-        float s = P[kOffB3 + i];
-#pragma unroll
-        for (int k = 0; k < kMlpH2; k++) s = fmaf(P[kOffW3 + i * kMlpH2 + k], h2[k], s);
-        o[i] = s;
-    }
-    return ok;
-}
-
-// ---- the same network on the matrix cores ------------------------------------
+// ---- network forward pass on the matrix cores -------------------------------
 // A wavefront evaluates the network for its 64 links as three bf16 GEMMs on
 // v_mfma_f32_16x16x16_bf16 (FP32 accumulation): weights are the A operand
 // (rows = units), activations the B operand (columns = links), 16 links per
@@ -296,6 +258,5 @@ template <class T> __device__ __forceinline__ void project_algebra(const M3<T> &
     g[7] = -0.5f * r3 * float(n.e[0][0].i + n.e[1][1].i - 2.f * n.e[2][2].i);
 }
 
-__device__ __forceinline__ float bf16_to_float(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
 
 } // namespace glacx
