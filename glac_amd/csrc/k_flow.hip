@@ -160,6 +160,22 @@ template <bool DB, class T> __device__ __forceinline__ M3<T> su3_product(const M
     return p;
 }
 
+// Order in which a launch visits the 32-site chunks of each parity.  In
+// storage order a slab's chunks run (lt, lz, y-chunk), so a chunk's
+// t-neighbours come b_z x (plane) chunks later, beyond an XCD's L2 on large
+// lattices.  When every x-y plane holds whole chunks (Ny Nx / 2 divisible by
+// 32) the launch instead runs (y-chunk, lz, lt) within each slab and parity:
+// z- and t-neighbours inside the slab are then at most b_z b_t chunks apart.
+__device__ __forceinline__ int flow_chunk(const Geom &g, int b)
+{
+    const int plane = g.run >> g.lbz;  // Ny Nx / 2 entries per x-y plane of one parity
+    if (plane & 31) return b;
+    const int cpp = plane >> 5, per = g.bh >> 5;  // chunks per plane, per slab-parity
+    const int sp = fdiv(b << 5, g.d_bh), r = b - sp * per;
+    const int lt = r & ((1 << g.lbt) - 1), lz = (r >> g.lbt) & ((1 << g.lbz) - 1), yc = r >> (g.lbt + g.lbz);
+    return sp * per + ((lt << g.lbz) + lz) * cpp + yc;
+}
+
 template <class T> struct FlowMeasLds {
     T f[6][8][kFlowSites];
 };
@@ -179,7 +195,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : 2) void f
     __shared__ FlowLds<T> sh;
     const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // one direction per wavefront
     const int lane = threadIdx.x & 63;
-    const int hb = xcd_block(blockIdx.x, gridDim.x);
+    const int hb = flow_chunk(g, xcd_block(blockIdx.x, gridDim.x));
     const int h_raw = hb * (kFlowSites / 2) + (lane & 31);
     const bool valid = h_raw < g.vh;
     const int h = valid ? h_raw : g.vh - 1;  // idle lanes of a partial block mirror a real site, store nothing
