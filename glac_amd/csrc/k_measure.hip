@@ -160,8 +160,17 @@ __global__ __launch_bounds__(kMeasSites * kMeasWaves, sizeof(T) == 8 ? 2 : 1) vo
 {
     constexpr bool SUPER = MODE == MEAS_SUPER, DENSITY = MODE == MEAS_DENSITY;
     const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int rest = fdiv(b, g.d_cpr), jc = b - rest * g.cpr;
-    const int lt = rest & ((1 << g.lbt) - 1), blk = rest >> g.lbt;
+    const int rest = fdiv(b, g.d_cpr);
+    int jc = b - rest * g.cpr, lt = rest & ((1 << g.lbt) - 1);
+    const int blk = rest >> g.lbt;
+    const int plane = g.run >> g.lbz;  // Ny Nx / 2 entries per x-y plane of one parity
+    if ((plane & 31) == 0) {
+        // visit the slab's chunks y-chunk-major, t fastest (as the flow stage
+        // does): z- and t-neighbours then at most b_z b_t chunks apart
+        const int r = b - (blk << g.lbt) * g.cpr, q = r >> g.lbt;
+        lt = r & ((1 << g.lbt) - 1);
+        jc = (q & ((1 << g.lbz) - 1)) * (plane >> 5) + (q >> g.lbz);
+    }
     const int bt = fdiv(blk, g.d_nzb), zb = blk - bt * g.nzb;
     const int t = (bt << g.lbt) + lt, c = zb * g.cpr + jc;
     const int lane = threadIdx.x & 63;
