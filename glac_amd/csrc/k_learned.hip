@@ -363,7 +363,14 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 3 : 1) void metropolis_lea
             }
             const T ds = T(-bf) * (retrace_mul(xm, w) - trw);
             if (accept_prob(ds) > rs.open()) {
-                u = mul(xm, u);
+                if constexpr (R == 2) {  // SU(3): two rows, third rebuilt
+                    M3<T> v;
+                    mm<false, false, false, 2>(v, xm, u);
+                    rebuild_third_row(v);
+                    u = v;
+                } else {
+                    u = mul(xm, u);
+                }
                 w = mul(xm, w);
                 trw = retrace(w);
                 accepted++;
@@ -372,7 +379,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 3 : 1) void metropolis_lea
         if (valid) {
             if (prm.reunit) {
                 M3<double> d = convert<double>(u);
-                reunitarize(d);
+                reunitarize<R>(d);  // R = 2: the store keeps rows 0 and 1 only
                 u = convert<T>(d);
             }
             store_link<T, R>(f, g, mu, idx, u);

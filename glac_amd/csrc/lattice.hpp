@@ -418,17 +418,31 @@ __device__ __forceinline__ M3<T> staple(const typename vec2<T>::type *__restrict
         const int ipn = index_of(g, step(g, x, nu, +1));
         const int imn = index_of(g, step(g, x, nu, -1));
         const int ipmmn = index_of(g, step(g, xpm, nu, -1));
-        {
-            const M3<T> u1 = load_link<T, R>(vnu, ipm);
-            const M3<T> u2 = load_link<T, R>(vmu, ipn);
-            const M3<T> u3 = load_link<T, R>(vnu, ix);
-            mm<false, true, true>(a, mul_ad(u1, u2), u3);
-        }
-        {
-            const M3<T> u1 = load_link<T, R>(vnu, ipmmn);
-            const M3<T> u2 = load_link<T, R>(vmu, imn);
-            const M3<T> u3 = load_link<T, R>(vnu, imn);
-            mm<false, false, true>(a, mul_dd(u1, u2), u3);
+        if constexpr (R == 2) {
+            // SU(3) links (the compressed mode checks them): each term's
+            // partial products keep two rows, the term's third is rebuilt
+            M3<T> p, t;
+            mm<false, true, false, 2>(p, load_rows01<T, R>(vnu, ipm), load_link<T, R>(vmu, ipn));
+            mm<false, true, false, 2>(t, p, load_link<T, R>(vnu, ix));
+            rebuild_third_row(t);
+            add_to(a, t);
+            mm<true, true, false, 2>(p, load_link<T, R>(vnu, ipmmn), load_link<T, R>(vmu, imn));
+            mm<false, false, false, 2>(t, p, load_link<T, R>(vnu, imn));
+            rebuild_third_row(t);
+            add_to(a, t);
+        } else {
+            {
+                const M3<T> u1 = load_link<T, R>(vnu, ipm);
+                const M3<T> u2 = load_link<T, R>(vmu, ipn);
+                const M3<T> u3 = load_link<T, R>(vnu, ix);
+                mm<false, true, true>(a, mul_ad(u1, u2), u3);
+            }
+            {
+                const M3<T> u1 = load_link<T, R>(vnu, ipmmn);
+                const M3<T> u2 = load_link<T, R>(vmu, imn);
+                const M3<T> u3 = load_link<T, R>(vnu, imn);
+                mm<false, false, true>(a, mul_dd(u1, u2), u3);
+            }
         }
     }
     return a;
