@@ -288,6 +288,29 @@ public:
         m_dev->check(glacx_measure(m_dev->ctx(), f, &o[0], &o[2], &o[1], &o[3]), "measure");
     }
 
+    // GLAC's System::flowConfiguration loop (GLAC/system.cpp:494-508) in one
+    // call: calculate(lattice, 0), then nFlows times {flowField; calculate},
+    // the lattice flowed in place and observations 0..nFlows filled.  With
+    // FP32 links each step's first RK3 stage measures the field it reads,
+    // which saves the separate clover pass of the step-by-step loop.
+    void flowConfiguration(L *lattice, unsigned int nFlows, double epsilon)
+    {
+        if (nFlows + 1 > m_obs.size()) fatal("flowConfiguration: more flow steps than observations");
+        const int f = m_dev->acquire(lattice);
+        const size_t m = nFlows + 1;
+        std::vector<double> p(m), e(m), q(m), qt(m * (size_t)m_nt);
+        m_dev->check(glacx_flow(m_dev->ctx(), f, (int)nFlows, epsilon, p.data(), e.data(), q.data(), qt.data()),
+                     "flow");
+        m_dev->markDeviceModified(lattice);
+        for (size_t i = 0; i < m; i++) {
+            std::vector<double> &o = m_obs[i];
+            o[0] = p[i];
+            o[1] = q[i];
+            o[2] = e[i];
+            std::copy(qt.begin() + i * m_nt, qt.begin() + (i + 1) * m_nt, o.begin() + 3);
+        }
+    }
+
     std::vector<double> getObservablesVector(unsigned int iObs) const { return m_obs[iObs]; }
     void copyObservable(unsigned int iObs, const std::vector<double> &obs) { m_obs[iObs] = obs; }
     double getObservable(unsigned int iObs) const { return m_obs[iObs][0]; }

@@ -4,7 +4,8 @@
 //   host_adapter_check cpu   GLAC's dS = -36 test on a unit field
 //                            (GLAC/tests/unit_tests/actiontests.cpp:67-114)
 //   host_adapter_check gpu   flowConfiguration-style loop (calculate, then
-//                            flowField + calculate per step) and Metropolis
+//                            flowField + calculate per step), the same loop
+//                            as one flowConfiguration call (FP64, FP32) and Metropolis
 //                            updates through the adapter, checked against
 //                            the FP64 oracle; device residency is checked by
 //                            counting host<->device transfers.
@@ -135,6 +136,33 @@ static int run_gpu(const std::string &dir)
             for (int k = 0; k < 18; k++) err = std::fmax(err, std::fabs(L[mu].m_sites[s].mat[k] - W[(mu * V + s) * 18 + k]));
     REQUIRE(err < 1e-12, "flowed field after syncHost err %g", err);
     std::printf("PASS flow + clover observables through Flow / MasterSamplerTopcXYZ\n");
+
+    // the same loop in one call, FP64 and FP32 links (the latter measuring
+    // inside each step's first stage), on fresh copies of the start field
+    for (int prec = 0; prec < 2; prec++) {
+        Lattice<SU3> F[4];
+        for (int mu = 0; mu < 4; mu++) {
+            F[mu].allocate(N);
+            for (unsigned long s = 0; s < V; s++)
+                for (int k = 0; k < 18; k++) F[mu].m_sites[s].mat[k] = U[(mu * V + s) * 18 + k];
+        }
+        glac_gpu::Device devf(N, prec == 0 ? GLACX_FP64 : GLACX_FP32_LINKS, 0);
+        glac_gpu::MasterSamplerTopcXYZ<Lattice<SU3>> obsf(&devf, 6);
+        obsf.flowConfiguration(F, 5, 0.02);
+        const double tp = prec == 0 ? 1e-12 : 5e-7, te = prec == 0 ? 1e-11 : 1e-5, tq = prec == 0 ? 1e-12 : 1e-4;
+        std::vector<double> Wf = U;
+        for (int i = 0; i <= 5; i++) {
+            if (i > 0) or_flow_step(Wf.data(), n, 0.02);
+            or_measure(Wf.data(), n, out, qt.data());
+            const std::vector<double> o = obsf.getObservablesVector(i);
+            REQUIRE(std::fabs(o[0] - out[0]) < tp, "flowConfiguration prec %d plaq step %d", prec, i);
+            REQUIRE(std::fabs(o[1] - out[2]) < tq, "flowConfiguration prec %d topc step %d", prec, i);
+            REQUIRE(std::fabs(o[2] / out[1] - 1.0) < te, "flowConfiguration prec %d energy step %d", prec, i);
+            for (int t = 0; t < 8; t++)
+                REQUIRE(std::fabs(o[3 + t] - qt[t]) < tq, "flowConfiguration prec %d topct step %d", prec, i);
+        }
+    }
+    std::printf("PASS flowConfiguration in one call (FP64, FP32)\n");
 
     // Metropolis through System::update's interface on a separate chain
     Lattice<SU3> M[4];

@@ -40,4 +40,4 @@ def test_flow_measure_metropolis_through_adapter(tmp_path):
     out = subprocess.run([exe, "gpu", str(tmp_path)], capture_output=True, text=True, timeout=120)
     print(out.stdout)
     assert out.returncode == 0, out.stdout + out.stderr
-    assert out.stdout.count("PASS") == 4
+    assert out.stdout.count("PASS") == 5
