@@ -920,7 +920,9 @@ double or_cb_sweep(double *U, const int n[4], double beta, double eps, int nhit,
  * i w.T with T = lambda/2 (checked entry by entry against the Gell-Mann
  * matrices) and projection is its adjoint (Re tr(i T_a Y(w)) = w_a / 2
  * orthogonality); the sweep samples exp(beta/3 Re tr(U A)) (single-link
- * histogram) and keeps GLAC's plaquette.
+ * histogram) and keeps GLAC's plaquette; or_mlp_forward rounds each layer's
+ * input to bf16 to nearest even, as the device's matrix-core GEMMs take it
+ * (an identity network separates this from truncation and from no rounding).
  */
 
 static const int STREAM_LEARNED = 3;
