@@ -144,7 +144,10 @@ int glacx_plaquette(glacx_ctx *ctx, int field, double *plaq);
  * non-NULL the clover observables are measured before the first step and
  * after every step, as System::flowConfiguration does
  * (GLAC/system.cpp:471-517): plaq, e_raw and q hold n_steps + 1 values and
- * q_t holds (n_steps + 1) * Nt values, row i = flow time i * eps. */
+ * q_t holds (n_steps + 1) * Nt values, row i = flow time i * eps.  With FP32
+ * links (and z-block runs of whole 32-site chunks, true for the usual even
+ * extents) each step's first RK3 stage measures the field it reads, so a
+ * measured flow costs little more than an unmeasured one. */
 int glacx_flow(glacx_ctx *ctx, int field, int n_steps, double eps,
                double *plaq, double *e_raw, double *q, double *q_t);
 
