@@ -145,9 +145,10 @@ int glacx_plaquette(glacx_ctx *ctx, int field, double *plaq);
  * after every step, as System::flowConfiguration does
  * (GLAC/system.cpp:471-517): plaq, e_raw and q hold n_steps + 1 values and
  * q_t holds (n_steps + 1) * Nt values, row i = flow time i * eps.  With FP32
- * links (and z-block runs of whole 32-site chunks, true for the usual even
- * extents) each step's first RK3 stage measures the field it reads, so a
- * measured flow costs little more than an unmeasured one. */
+ * links, when b_z Ny Nx / 2 is a multiple of 32 (b_z = 4 if 4 divides Nz,
+ * else 2; so Nx Ny divisible by 16, or by 32 when 4 does not divide Nz),
+ * each step's first RK3 stage measures the field it reads, so a measured
+ * flow costs little more than an unmeasured one. */
 int glacx_flow(glacx_ctx *ctx, int field, int n_steps, double eps,
                double *plaq, double *e_raw, double *q, double *q_t);
 
