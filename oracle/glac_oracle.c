@@ -25,9 +25,8 @@
  *      C++11 [rand.predef] mt19937_64 (10000th output 9981545732273789042)
  *      and libstdc++ uniform_real_distribution (compared with g++'s <random>);
  *      Philox4x32-10 of Salmon et al., SC'11 (Random123 known-answer
- *      vectors); xoshiro128** of Blackman & Vigna (parity unpinned against
- *      an external vector: it is only the shared stream definition of the
- *      device and oracle checkerboard sweeps).
+ *      vectors); xoshiro128** 1.1 of Blackman & Vigna (first ten outputs
+ *      from the state (1, 2, 3, 4) of their reference code).
  *  or_staple / or_delta_action   GLAC/actions/wilsongaugeaction.cpp:9, 24-53.
  *      Pinned: unit field with U' = 2*1, beta = 6 gives dS = -36 at every
  *      link (GLAC/tests/unit_tests/actiontests.cpp:67-114); staple of a pure

@@ -8,9 +8,10 @@
   GLAC/math/matrices/su3matrixgenerator.cpp:22-24).
 - Philox4x32-10: known-answer vectors published with Random123 (Salmon et
   al., SC'11).
-- xoshiro128**: no external vector is held here; only its output statistics
-  are checked (it is the shared stream definition of the device and oracle
-  checkerboard sweeps, so parity between them does not depend on it).
+- xoshiro128** 1.1: the first ten outputs from the state (1, 2, 3, 4), as
+  produced by Blackman & Vigna's reference C code (the same vector is the
+  known-answer test of the Rust rand_xoshiro crate); its output statistics are
+  checked as well.
 """
 import os
 import shutil
@@ -69,6 +70,13 @@ def test_uniform_real_distribution_matches_libstdcxx():
 ])
 def test_philox4x32_10_known_answers(ctr, key, want):
     assert list(O.philox4x32_10(ctr, key)) == want
+
+
+def test_xoshiro128ss_known_answer():
+    st = np.array([1, 2, 3, 4], dtype=np.uint32)
+    want = [11520, 0, 5927040, 70819200, 2031721883, 1637235492,
+            1287239034, 3734860849, 3729100597, 4258142804]
+    assert [int(v) for v in O.xoshiro128ss(st, 10)] == want
 
 
 def test_xoshiro128ss_output_is_uniform():
