@@ -340,8 +340,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 3 : 1) void metropolis_lea
         const double bf = prm.beta / 3.0;
         const T eps = T(prm.eps);
         const T x0abs = sqrt(T(1) - eps * eps);
-        M3<T> w = mul(u, a);
-        T trw = retrace(w);
+        T tra = retrace_mul(u, a);
         int accepted = 0;
 #pragma unroll 1
         for (int hit = 0; hit < prm.n_hit; hit++) {
@@ -361,18 +360,12 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 3 : 1) void metropolis_lea
             } else {
                 xm = rst_proposal<T>(rs, x0abs, eps);
             }
-            const T ds = T(-bf) * (retrace_mul(xm, w) - trw);
+            const M3<T> v = propose_link<T, R>(xm, u);
+            const T trv = retrace_mul(v, a);
+            const T ds = T(-bf) * (trv - tra);
             if (accept_prob(ds) > rs.open()) {
-                if constexpr (R == 2) {  // SU(3): two rows, third rebuilt
-                    M3<T> v;
-                    mm<false, false, false, 2>(v, xm, u);
-                    rebuild_third_row(v);
-                    u = v;
-                } else {
-                    u = mul(xm, u);
-                }
-                w = mul(xm, w);
-                trw = retrace(w);
+                u = v;
+                tra = trv;
                 accepted++;
             }
         }

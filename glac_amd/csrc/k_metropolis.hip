@@ -13,8 +13,8 @@
 // blocks (0,1), (0,2), (1,2), x0 = sign(r0) sqrt(1 - eps^2),
 // x = eps r / |r|, r ~ U(-1/2, 1/2)^4, product R S T or its inverse by a coin.
 // dS = -(beta/3) Re tr[(U' - U) A] (wilsongaugeaction.cpp:9, 24-27) is
-// evaluated as -(beta/3) (Re tr(X W) - Re tr W) with W = U A kept up to date;
-// accept iff exp(-dS) > u (GLAC/system.cpp:347).  After the hits the link is
+// evaluated as -(beta/3) (Re tr(U' A) - Re tr(U A)), with Re tr(U A) carried
+// from the last accepted hit; accept iff exp(-dS) > u (GLAC/system.cpp:347).  After the hits the link is
 // optionally reunitarised in FP64.  Uniforms come from the link's own
 // counter-based stream (rng.hpp), 13 centred ones for the proposal and one
 // open one for the test per hit, in that order.
@@ -39,24 +39,16 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 2 : 1) void metropolis_ker
         const T eps = T(prm.eps);
         const T x0abs = sqrt(T(1) - eps * eps);
         const double bf = prm.beta / 3.0;
-        M3<T> w = mul(u, a);
-        T trw = retrace(w);
+        T tra = retrace_mul(u, a);
         int accepted = 0;
 #pragma unroll 1
         for (int hit = 0; hit < prm.n_hit; hit++) {
-            const M3<T> xm = rst_proposal<T>(rs, x0abs, eps);
-            const T ds = T(-bf) * (retrace_mul(xm, w) - trw);
+            const M3<T> v = propose_link<T, R>(rst_proposal<T>(rs, x0abs, eps), u);
+            const T trv = retrace_mul(v, a);
+            const T ds = T(-bf) * (trv - tra);
             if (accept_prob(ds) > rs.open()) {
-                if constexpr (R == 2) {  // SU(3): two rows, third rebuilt
-                    M3<T> v;
-                    mm<false, false, false, 2>(v, xm, u);
-                    rebuild_third_row(v);
-                    u = v;
-                } else {
-                    u = mul(xm, u);
-                }
-                w = mul(xm, w);
-                trw = retrace(w);
+                u = v;
+                tra = trv;
                 accepted++;
             }
         }

@@ -80,6 +80,20 @@ template <class T> __device__ __forceinline__ M3<T> dagger(const M3<T> &m)
     return d;
 }
 
+// The proposed link U' = X U.  With two stored rows (SU(3) links) only rows 0
+// and 1 are multiplied and row 2 is rebuilt as conj(row0 x row1).
+template <class T, int R> __device__ __forceinline__ M3<T> propose_link(const M3<T> &xm, const M3<T> &u)
+{
+    if constexpr (R == 2) {
+        M3<T> v;
+        mm<false, false, false, 2>(v, xm, u);
+        rebuild_third_row(v);
+        return v;
+    } else {
+        return mul(xm, u);
+    }
+}
+
 template <class T> __device__ __forceinline__ M3<T> rst_proposal(Stream &rs, T x0abs, T eps)
 {
     cpx<T> ra, rb, sa, sb, ta, tb;
