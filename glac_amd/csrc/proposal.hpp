@@ -12,6 +12,13 @@
 
 namespace glacx {
 
+// 1/sqrt for the proposal's normalisations: the hardware estimate (about
+// 1 ulp) for FP32 links, whose products round at that level anyway; the
+// correctly rounded library form for FP64 links.  (HIP's rsqrtf evaluates
+// through FP64 to round correctly, about ten instructions per call.)
+__device__ __forceinline__ float proposal_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double proposal_rsqrt(double x) { return rsqrt(x); }
+
 // SU(2) near unity from 4 centred uniforms; returns (a, b) with
 // matrix [[a, b], [-conj(b), conj(a)]], a = x0 + i x3, b = x2 + i x1.
 template <class T>
@@ -19,10 +26,10 @@ __device__ __forceinline__ void su2_near_unity(const T r0, const T r1, const T r
                                                cpx<T> &a, cpx<T> &b)
 {
     T x0 = r0 < T(0) ? -x0abs : x0abs;
-    const T sc = eps * rsqrt(fma(r1, r1, fma(r2, r2, r3 * r3)));  // eps / |r|
+    const T sc = eps * proposal_rsqrt(fma(r1, r1, fma(r2, r2, r3 * r3)));  // eps / |r|
     T x1 = r1 * sc, x2 = r2 * sc, x3 = r3 * sc;
     // |x| = 1 up to rounding (x0^2 = 1 - eps^2); renormalise as GLAC does
-    const T inv = rsqrt(fma(x0, x0, fma(x1, x1, fma(x2, x2, x3 * x3))));
+    const T inv = proposal_rsqrt(fma(x0, x0, fma(x1, x1, fma(x2, x2, x3 * x3))));
     x0 *= inv;
     x1 *= inv;
     x2 *= inv;
