@@ -372,7 +372,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 3 : 1) void metropolis_lea
         if (valid) {
             if (prm.reunit) {
                 M3<double> d = convert<double>(u);
-                reunitarize<R>(d);  // R = 2: the store keeps rows 0 and 1 only
+                reunitarize<R, sizeof(T) == 4>(d);  // R = 2: the store keeps rows 0 and 1 only
                 u = convert<T>(d);
             }
             store_link<T, R>(f, g, mu, idx, u);
