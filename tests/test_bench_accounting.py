@@ -29,3 +29,12 @@ def test_stage_bytes_per_step_and_per_launch():
 
 def test_metropolis_bytes():
     assert bench.metropolis_bytes_per_launch(V, True, 2) == 4.5 * V * 48
+
+
+def test_more_than_one_gpu_is_refused():
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
