@@ -553,7 +553,6 @@ int glacx_metropolis(glacx_ctx *c, int n_sweeps, double beta, double eps, int n_
     int rc = ensure(c, &c->counts, &c->counts_len, (size_t)n_sweeps);
     if (rc) return rc;
     HIP_TRY(c, hipMemsetAsync(c->counts, 0, sizeof(unsigned long long) * n_sweeps, c->stream));
-    const size_t row = (size_t)3 + c->dims[3];
     if (plaq && (rc = ensure(c, &c->results, &c->results_len, (size_t)n_sweeps))) return rc;
     for (int i = 0; i < n_sweeps; i++) {
         MetroParams p;
@@ -566,7 +565,6 @@ int glacx_metropolis(glacx_ctx *c, int n_sweeps, double beta, double eps, int n_
         if ((rc = DISPATCH(c, run_metropolis_sweep, c, p, c->counts + i))) return rc;
         if (plaq && (rc = DISPATCH(c, run_plaquette, c, c->field[GLACX_FIELD_MAIN], c->results + i))) return rc;
     }
-    (void)row;
     std::vector<unsigned long long> counts(n_sweeps);
     HIP_TRY(c, hipMemcpyAsync(counts.data(), c->counts, sizeof(unsigned long long) * n_sweeps, hipMemcpyDeviceToHost,
                               c->stream));
