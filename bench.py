@@ -18,10 +18,13 @@ Also reported, each from its own timed region:
    measurement fused into stage 0 (FP32 links), that is RK3 stages 1 and 2;
    ``stage0_measure`` reports the fused stage 0 the same way;
  - ``indep_configs_per_s``: effectively independent configurations per second,
-   configs/s / (2 tau_int(Q^2)), for the learned proposal with FP32 links and
-   for GLAC's plain RST proposal with FP64 links (``--indep-seconds`` each);
- - ``cpu_baseline``: the FP64 oracle (one host thread) doing one flow step plus
-   one measurement on the same 32^4 lattice.
+   configs/s / (2 tau_int(Q^2)), for the learned proposal with FP32 links,
+   GLAC's RST proposal with the same FP32 links (the network's gain at equal
+   precision) and GLAC's plain RST proposal with FP64 links
+   (``--indep-seconds`` each);
+ - ``cpu_baseline``: the FP64 oracle (its site loops on the host's OpenMP
+   threads) doing flow steps, each with one measurement, on the same 32^4
+   lattice.
 
 Run ``python bench.py [--steps K] [--warmup W]``; one JSON line on stdout.
 """
@@ -76,10 +79,24 @@ def profiled_traffic(n, kernel):
     return entry["traffic_bytes_per_launch"] if entry else None
 
 
-def independent_configs(G, n, beta, seconds):
+VARIANTS = {"fp32c_learned": ("FP32_COMPRESSED", "learned"), "fp32c_rst": ("FP32_COMPRESSED", "rst"),
+            "fp64_plain": ("FP64", "rst")}
+
+
+def ratio_with_error(a, b):
+    """indep. configs/s of a over b, with the error propagated from the two
+    tau_int(Q^2) estimates (the configs/s rates are measured, not sampled)."""
+    ratio = a["indep_configs_per_s"] / b["indep_configs_per_s"]
+    rel = np.hypot(a["tau_int_q2_err"] / a["tau_int_q2"], b["tau_int_q2_err"] / b["tau_int_q2"])
+    return ratio, ratio * rel
+
+
+def independent_configs(G, n, beta, seconds, variants=tuple(VARIANTS)):
     """Effectively independent configurations per second,
-    configs/s / (2 tau_int(Q^2)), for the learned proposal with FP32 links
-    (compressed) and GLAC's fixed-eps RST proposal with FP64 links.
+    configs/s / (2 tau_int(Q^2)), for the learned proposal with compressed
+    FP32 links, GLAC's fixed-eps RST proposal with the same FP32 links (the
+    network's gain at equal precision) and GLAC's RST proposal with FP64
+    links (the plain FP64 path).
 
     Each variant: Haar hot start, 1500 RST sweeps; the learned variant then
     collects 20 sweeps of (staple, link) samples and trains its network
@@ -90,7 +107,8 @@ def independent_configs(G, n, beta, seconds):
     from glac_amd.system import EnsembleConfig, run_ensemble
     from glac_amd.proposal import train_from_chain
     out = {}
-    for name, prec, proposal in [("fp32c_learned", G.FP32_COMPRESSED, "learned"), ("fp64_plain", G.FP64, "rst")]:
+    for name in variants:
+        prec, proposal = getattr(G, VARIANTS[name][0]), VARIANTS[name][1]
         L = G.Lattice(n, prec)
         L.hot_start(31)
         L.metropolis(1500, beta, 0.24, 10, seed=3)
@@ -118,11 +136,14 @@ def independent_configs(G, n, beta, seconds):
                      "acceptance": r.acceptance, "train_seconds": train_s}
         log(f"[bench] {name}: {r.configs_per_second:.2f} configs/s, tau_int(Q^2) {ti['tau_int']:.2f} "
             f"+- {ti['dtau']:.2f}, {rate:.3f} indep. configs/s, <Q^2> {np.mean(r.q ** 2):.2f}")
-    a, b = out["fp32c_learned"], out["fp64_plain"]
-    ratio = a["indep_configs_per_s"] / b["indep_configs_per_s"]
-    rel = np.hypot(a["tau_int_q2_err"] / a["tau_int_q2"], b["tau_int_q2_err"] / b["tau_int_q2"])
-    out["ratio_learned_fp32_over_plain_fp64"] = ratio
-    out["ratio_err"] = ratio * rel
+    if "fp32c_learned" in out and "fp64_plain" in out:
+        r, e = ratio_with_error(out["fp32c_learned"], out["fp64_plain"])
+        out["ratio_learned_fp32_over_plain_fp64"], out["ratio_err"] = r, e
+    if "fp32c_learned" in out and "fp32c_rst" in out:
+        r, e = ratio_with_error(out["fp32c_learned"], out["fp32c_rst"])
+        out["same_precision"] = {"ratio_learned_over_rst_fp32c": r, "ratio_err": e,
+                                 "tau_int_q2_ratio_rst_over_learned":
+                                     out["fp32c_rst"]["tau_int_q2"] / out["fp32c_learned"]["tau_int_q2"]}
     out["protocol"] = ("NCor 50 sweeps (10 hits, eps 0.24 fallback/RST), flow 50 x 0.02, Q at t = 1; "
                        "tau_int(Q^2) by Wolff's Gamma method (S = 1.5), errors from its estimate")
     # the same measurement run for several minutes per variant (committed), for comparison
@@ -133,9 +154,11 @@ def independent_configs(G, n, beta, seconds):
             r = json.load(f)
         out["long_run"] = {"ratio_learned_fp32_over_plain_fp64": r["ratio_learned_fp32_over_plain_fp64"],
                            "ratio_err": r["ratio_err"], "seconds_per_variant": r.get("seconds_per_variant"),
-                           "fp32c_learned_indep_configs_per_s": r["fp32c_learned"]["indep_configs_per_s"],
-                           "fp64_plain_indep_configs_per_s": r["fp64_plain"]["indep_configs_per_s"],
+                           "same_precision": r.get("same_precision"),
                            "source": "profiles/" + os.path.basename(ref)}
+        for k in VARIANTS:
+            if k in r:
+                out["long_run"][k + "_indep_configs_per_s"] = r[k]["indep_configs_per_s"]
     return out
 
 
@@ -150,39 +173,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--dims", default="32,32,32,32")
     ap.add_argument("--sweeps", type=int, default=100, help="timed Metropolis sweeps")
+    ap.add_argument("--therm", type=int, default=1000,
+                    help="thermalising Metropolis sweeps from the hot start before anything is timed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=4, help="flow steps of the CPU baseline sample")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--precision", choices=["fp32", "fp32c"], default="fp32c",
                     help="FP32 mode of the timed flow and sweeps: fp32 stores 3 rows per link, fp32c two")
-    ap.add_argument("--indep-seconds", type=float, default=45.0,
+    ap.add_argument("--indep-seconds", type=float, default=30.0,
                     help="wall seconds of ensemble generation per variant for the independent-configs/s "
-                         "measurement (learned FP32 vs plain FP64); 0 skips it")
+                         "measurement (learned FP32, RST FP32, RST FP64); 0 skips it")
     ap.add_argument("--only-indep", action="store_true",
                     help="run only the independent-configs/s measurement and print its JSON")
-    ap.add_argument("--calibration", action="store_true",
-                    help="only run glac_amd/calibrate (FP32/FP64 FMA issue rate, L1 bandwidth) and print its JSON")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="only run a 1 GiB device-to-device copy kernel (for scaling the "
-                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
-
-    if args.calibration:
-        import subprocess
-        exe = os.path.join(ROOT, "glac_amd", "calibrate")
-        if not os.path.exists(exe):
-            sys.exit(f"{exe} is not built; run __graft_entry__.build()")
-        sys.exit(subprocess.run([exe]).returncode)
-
-    if args.copy_calibration:
-        import torch
-        src = torch.ones(2**28, dtype=torch.float32, device="cuda:0")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes_read": src.numel() * 4,
-                          "copy_calibration_bytes_written": dst.numel() * 4, "launches": 3}))
-        return
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -205,13 +208,13 @@ def main():
     L = G.Lattice(n, G.FP32_COMPRESSED if rows == 2 else G.FP32_LINKS)
     L.hot_start(20261202)
     log(f"[bench] lattice {n}, FP32 links ({rows} stored rows), {L.device_bytes / 2**20:.0f} MiB on device")
-    # Metropolis: a few thermalising sweeps, then the timed sweeps
-    L.metropolis(5, beta, 0.24, 10, reunit_every=1, seed=7, sweep_offset=0)
+    # Metropolis: thermalising sweeps, then the timed sweeps
+    L.metropolis(args.therm, beta, 0.24, 10, reunit_every=1, seed=7, sweep_offset=0)
     L.synchronize()
     L.timing(True)
     L.timing_reset()
     t0 = time.perf_counter()
-    acc = L.metropolis(args.sweeps, beta, 0.24, 10, reunit_every=1, seed=7, sweep_offset=5)
+    acc = L.metropolis(args.sweeps, beta, 0.24, 10, reunit_every=1, seed=7, sweep_offset=args.therm)
     L.synchronize()
     t_sweeps = time.perf_counter() - t0
     m_launches, m_ms = L.timing_read(G.K_METROPOLIS)
@@ -243,6 +246,7 @@ def main():
     fm_launches, fm_ms = L.timing_read(G.K_FLOW_STAGE_MEASURE)
     meas_launches, meas_ms = L.timing_read(G.K_MEASURE)
     L.timing(False)
+    U_therm = None if args.no_fp64 else L.download(G.MAIN)
     L.close()
     steps_per_s = args.steps / elapsed
     log(f"[bench] flow {args.precision}: {args.steps} steps in {elapsed:.3f} s = {steps_per_s:.1f} steps/s "
@@ -266,7 +270,8 @@ def main():
     fp64 = None
     if not args.no_fp64:
         L = G.Lattice(n, G.FP64)
-        L.hot_start(20261202, field=G.FLOW)
+        L.upload(U_therm, field=G.FLOW)       # the same thermalised configuration
+        del U_therm
         k64 = max(args.steps // 3, 10)
         L.flow(min(args.warmup, 10), eps_flow, field=G.FLOW)
         L.synchronize()
@@ -296,14 +301,18 @@ def main():
     if not args.no_cpu_baseline:
         import oracle as O
         U = O.hot_start_haar(n, 1)
+        cpu_steps = args.cpu_steps
         t0 = time.perf_counter()
-        O.flow_step(U, n, eps_flow)
-        O.measure(U, n)
+        for _ in range(cpu_steps):
+            O.flow_step(U, n, eps_flow)
+            O.measure(U, n)
         cpu_s = time.perf_counter() - t0
-        cpu = {"value": 1.0 / cpu_s, "unit": "flow steps/s", "cores": 1, "kind": "port",
-               "sample": f"one RK3 step + one clover measurement of the FP64 oracle on the full "
-                         f"{'x'.join(map(str, n))} lattice, one host thread ({cpu_s:.1f} s)"}
-        log(f"[bench] cpu baseline: {cpu_s:.2f} s per step")
+        threads = O.num_threads()
+        cpu = {"value": cpu_steps / cpu_s, "unit": "flow steps/s", "cores": threads, "kind": "port",
+               "sample": f"{cpu_steps} RK3 steps, each followed by a clover measurement, of the FP64 oracle "
+                         f"on the full {'x'.join(map(str, n))} lattice, site loops on {threads} host threads "
+                         f"(OpenMP), {cpu_s:.1f} s"}
+        log(f"[bench] cpu baseline: {cpu_s / cpu_steps:.2f} s per step on {threads} threads")
 
     out = {
         "metric": METRIC,
@@ -318,7 +327,8 @@ def main():
         "vs_baseline": None,
         "dtype": ("fp32 links (2 stored rows, third rebuilt)" if rows == 2 else "fp32 links")
                  + ", fp64 exp coefficients and sums",
-        "data": "synthetic: Haar hot start + 105 Metropolis sweeps at beta 6.1, seed 20261202",
+        "data": f"synthetic: Haar hot start + {args.therm} thermalising Metropolis sweeps at beta 6.1, "
+                f"seed 20261202; flow timed from flow time {args.warmup * eps_flow:g}",
         "config": {"workload": "RK3 Wilson flow step + clover measurement", "lattice": list(n),
                    "beta": beta, "flow_eps": eps_flow, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

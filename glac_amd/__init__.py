@@ -19,12 +19,14 @@ import os
 import numpy as np
 
 __all__ = ["Lattice", "GlacxError", "library", "FP64", "FP32_LINKS", "FP32_COMPRESSED", "MAIN", "FLOW",
+           "HOT_HAAR", "HOT_GLAC_RANDOM", "HOT_RST",
            "K_FLOW_STAGE", "K_MEASURE", "K_METROPOLIS", "K_PLAQUETTE", "K_METROPOLIS_LEARNED", "K_TRAIN",
            "K_FLOW_STAGE_MEASURE",
            "PROPOSAL_NPARAM", "LIB_PATH"]
 
 FP64, FP32_LINKS, FP32_COMPRESSED = 0, 1, 2
 MAIN, FLOW = 0, 1
+HOT_HAAR, HOT_GLAC_RANDOM, HOT_RST = 0, 1, 2
 K_FLOW_STAGE, K_MEASURE, K_METROPOLIS, K_PLAQUETTE, K_METROPOLIS_LEARNED, K_TRAIN = 0, 1, 2, 3, 4, 5
 K_FLOW_STAGE_MEASURE = 6  # flow stage 0 with the clover measurement fused
 PROPOSAL_NPARAM = 2468
@@ -67,7 +69,7 @@ def library():
         "glacx_measure_density": ([_vp, C.c_int, _dp, _dp, _dp, _dp, _dp], C.c_int),
         "glacx_flow_super": ([_vp, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int),
         "glacx_cold_start": ([_vp, C.c_int], C.c_int),
-        "glacx_hot_start": ([_vp, C.c_int, C.c_uint64], C.c_int),
+        "glacx_hot_start": ([_vp, C.c_int, C.c_uint64, C.c_int, C.c_double], C.c_int),
         "glacx_metropolis": ([_vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_uint64,
                               C.c_uint64, _dp, _dp], C.c_int),
         "glacx_measure": ([_vp, C.c_int, _dp, _dp, _dp, _dp], C.c_int),
@@ -185,8 +187,12 @@ class Lattice:
     def cold_start(self, field=MAIN):
         self._check(self._L.glacx_cold_start(self._h, field), "cold_start")
 
-    def hot_start(self, seed, field=MAIN):
-        self._check(self._L.glacx_hot_start(self._h, field, seed), "hot_start")
+    def hot_start(self, seed, field=MAIN, kind=None, eps=0.24):
+        """GLAC's hot start from the link streams of `seed`: kind HOT_HAAR
+        (default), HOT_GLAC_RANDOM (GLAC's generateRandom, det = -1) or
+        HOT_RST (GLAC's RSTHotStart with step eps)."""
+        kind = HOT_HAAR if kind is None else kind
+        self._check(self._L.glacx_hot_start(self._h, field, seed, kind, eps), "hot_start")
 
     # ------------------------------------------------------------ compute
     def metropolis(self, n_sweeps, beta, eps=0.24, n_hit=10, reunit_every=1, seed=1,

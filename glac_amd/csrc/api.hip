@@ -534,11 +534,18 @@ int glacx_cold_start(glacx_ctx *c, int field)
     return GLACX_OK;
 }
 
-int glacx_hot_start(glacx_ctx *c, int field, uint64_t seed)
+int glacx_hot_start(glacx_ctx *c, int field, uint64_t seed, int kind, double eps)
 {
     CTX_CHECK(c);
     if (!valid_field(field)) return fail(c, GLACX_E_ARG, "hot_start: bad field");
-    DISPATCH(c, launch_hot_start, c->field[field], c->geom, seed, c->stream);
+    if (kind != GLACX_HOT_HAAR && kind != GLACX_HOT_GLAC_RANDOM && kind != GLACX_HOT_RST)
+        return fail(c, GLACX_E_ARG, "hot_start: unknown kind");
+    if (kind != GLACX_HOT_HAAR && !(eps > 0.0 && eps < 1.0))
+        return fail(c, GLACX_E_ARG, "hot_start: need 0 < eps < 1");
+    if (kind == GLACX_HOT_GLAC_RANDOM && c->precision == GLACX_FP32_COMPRESSED)
+        return fail(c, GLACX_E_ARG, "hot_start: GLAC's random start gives det U = -1 links, which the compressed "
+                                    "FP32 mode (two stored rows, det = +1) cannot hold; use GLACX_FP32_LINKS");
+    DISPATCH(c, launch_hot_start, c->field[field], c->geom, seed, kind, eps, c->stream);
     CHECK_LAUNCH(c);
     return GLACX_OK;
 }

@@ -2,12 +2,14 @@
 //
 //  pack/unpack: GLAC host layout (AoS FP64, mu-major or .bin site-major,
 //               GLAC/io/fieldio.cpp:95-107, 226-235) <-> device SoA planes.
-//  hot start:   Haar SU(3) from the counter-based stream (replaces
-//               GLAC/system.cpp:200-214; GLAC's own generator makes det = -1
-//               matrices, su3matrixgenerator.h:156-161).
+//  hot start:   from the counter-based stream (GLAC/system.cpp:200-214):
+//               Haar SU(3), GLAC's "fully random" generateRandom (det = -1,
+//               su3matrixgenerator.h:98-164) or its RST matrix near unity
+//               (RSTHotStart, su3matrixgenerator.h:170-196).
 //  action derivative: Z_mu for GLAC's getActionDerivative API
 //               (GLAC/actions/wilsongaugeaction.cpp:55-86).
 #include "kernels.hpp"
+#include "proposal.hpp"
 #include "rng.hpp"
 
 namespace glacx {
@@ -48,10 +50,53 @@ __global__ __launch_bounds__(kBlock) void unpack_kernel(const typename vec2<T>::
     }
 }
 
-// One thread per link: 9 complex Gaussians by Box-Muller (entry k from stream
-// uniforms 2k, 2k+1), then FP64 row Gram-Schmidt with row 2 = conj(r0 x r1).
+// GLAC's generateRandom (GLAC/math/matrices/su3matrixgenerator.h:98-164) on
+// the entries already in m: column 0 normalised, column 1 orthogonalised to it
+// and normalised, column 2 = -conj(c0 x c1), which makes det = -1.
+__device__ __forceinline__ void glac_random_columns(M3<double> &m)
+{
+    double n0 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) n0 += m.e[i][0].r * m.e[i][0].r + m.e[i][0].i * m.e[i][0].i;
+    n0 = sqrt(n0);
+#pragma unroll
+    for (int i = 0; i < 3; i++) m.e[i][0] = {m.e[i][0].r / n0, m.e[i][0].i / n0};
+    double pr = 0.0, pi = 0.0;  // c0^dagger c1
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        pr += m.e[i][0].r * m.e[i][1].r + m.e[i][0].i * m.e[i][1].i;
+        pi += m.e[i][0].r * m.e[i][1].i - m.e[i][0].i * m.e[i][1].r;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double cr = m.e[i][0].r, ci = m.e[i][0].i;
+        m.e[i][1].r -= cr * pr - ci * pi;
+        m.e[i][1].i -= cr * pi + ci * pr;
+    }
+    double n1 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) n1 += m.e[i][1].r * m.e[i][1].r + m.e[i][1].i * m.e[i][1].i;
+    n1 = sqrt(n1);
+#pragma unroll
+    for (int i = 0; i < 3; i++) m.e[i][1] = {m.e[i][1].r / n1, m.e[i][1].i / n1};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int j = (i + 1) % 3, k = (i + 2) % 3;
+        const cpx<double> x = cmul(m.e[j][0], m.e[k][1]), y = cmul(m.e[k][0], m.e[j][1]);
+        m.e[i][2] = {-(x.r - y.r), x.i - y.i};
+    }
+}
+
+// One thread per link, drawing from the link's stream (seed, lex, mu, hot
+// start, 0):
+//   HOT_HAAR:  9 complex Gaussians by Box-Muller (entry k from uniforms 2k,
+//              2k+1), then FP64 row Gram-Schmidt with row 2 = conj(r0 x r1);
+//   HOT_GLAC_RANDOM:  18 uniforms eps (-1, 1) (entry k: 2k real, 2k+1
+//              imaginary), then glac_random_columns;
+//   HOT_RST:   GLAC's RST matrix with step eps (the Metropolis proposal's draws).
 template <class T, int R>
-__global__ __launch_bounds__(kBlock) void hot_start_kernel(typename vec2<T>::type *__restrict__ f, Geom g, uint64_t seed)
+__global__ __launch_bounds__(kBlock) void hot_start_kernel(typename vec2<T>::type *__restrict__ f, Geom g, uint64_t seed,
+                                                           int kind, double eps)
 {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= 4 * g.v) return;
@@ -59,15 +104,26 @@ __global__ __launch_bounds__(kBlock) void hot_start_kernel(typename vec2<T>::typ
     const int lex = lex_of(g, decode(g, idx));
     Stream rs(seed, (uint32_t)lex, (uint32_t)mu, STREAM_HOT_START, 0);
     M3<double> m;
+    if (kind == HOT_GLAC_RANDOM) {
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-        const double u1 = rs.open(), u2 = rs.open();
-        const double r = sqrt(-2.0 * log(u1));
-        double s, c;
-        sincos(2.0 * M_PI * u2, &s, &c);
-        m.e[k / 3][k % 3] = {r * c, r * s};
+        for (int k = 0; k < 9; k++) {
+            const double re = eps * rs.symmetric();
+            m.e[k / 3][k % 3] = {re, eps * rs.symmetric()};
+        }
+        glac_random_columns(m);
+    } else if (kind == HOT_RST) {
+        m = rst_proposal<double>(rs, sqrt(1.0 - eps * eps), eps);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const double u1 = rs.open(), u2 = rs.open();
+            const double r = sqrt(-2.0 * log(u1));
+            double s, c;
+            sincos(2.0 * M_PI * u2, &s, &c);
+            m.e[k / 3][k % 3] = {r * c, r * s};
+        }
+        reunitarize(m);
     }
-    reunitarize(m);
     store_link_any<T, R>(f, g, mu, idx, convert<T>(m));
 }
 
@@ -108,9 +164,9 @@ template <class T, int R> void launch_unpack(const void *f, double *stage, const
 {
     unpack_kernel<T, R><<<blocks_for(g.v), kBlock, 0, s>>>((const typename vec2<T>::type *)f, stage, g, site_major);
 }
-template <class T, int R> void launch_hot_start(void *f, const Geom &g, uint64_t seed, hipStream_t s)
+template <class T, int R> void launch_hot_start(void *f, const Geom &g, uint64_t seed, int kind, double eps, hipStream_t s)
 {
-    hot_start_kernel<T, R><<<blocks_for(4L * g.v), kBlock, 0, s>>>((typename vec2<T>::type *)f, g, seed);
+    hot_start_kernel<T, R><<<blocks_for(4L * g.v), kBlock, 0, s>>>((typename vec2<T>::type *)f, g, seed, kind, eps);
 }
 template <class T, int R> void launch_cold_start(void *f, const Geom &g, hipStream_t s)
 {
@@ -124,7 +180,7 @@ template <class T, int R> void launch_action_derivative(const void *f, double *s
 #define INSTANTIATE(T, R)                                                                                \
     template void launch_pack<T, R>(const double *, void *, const Geom &, int, hipStream_t);             \
     template void launch_unpack<T, R>(const void *, double *, const Geom &, int, hipStream_t);           \
-    template void launch_hot_start<T, R>(void *, const Geom &, uint64_t, hipStream_t);                   \
+    template void launch_hot_start<T, R>(void *, const Geom &, uint64_t, int, double, hipStream_t);      \
     template void launch_cold_start<T, R>(void *, const Geom &, hipStream_t);                            \
     template void launch_action_derivative<T, R>(const void *, double *, const Geom &, int, hipStream_t);
 INSTANTIATE(float, 3)

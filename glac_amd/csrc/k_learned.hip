@@ -372,7 +372,10 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 3 : 1) void metropolis_lea
         if (valid) {
             if (prm.reunit) {
                 M3<double> d = convert<double>(u);
-                reunitarize<R, sizeof(T) == 4>(d);  // R = 2: the store keeps rows 0 and 1 only
+                if constexpr (R == 2)
+                    reunitarize<2, true>(d);  // the store keeps rows 0 and 1 only
+                else
+                    reunitarize_keep_det_sign<sizeof(T) == 4>(d);  // det U = +1 or -1 kept
                 u = convert<T>(d);
             }
             store_link<T, R>(f, g, mu, idx, u);

@@ -14,10 +14,12 @@
 // x = eps r / |r|, r ~ U(-1/2, 1/2)^4, product R S T or its inverse by a coin.
 // dS = -(beta/3) Re tr[(U' - U) A] (wilsongaugeaction.cpp:9, 24-27) is
 // evaluated as -(beta/3) (Re tr(U' A) - Re tr(U A)), with Re tr(U A) carried
-// from the last accepted hit; accept iff exp(-dS) > u (GLAC/system.cpp:347).  After the hits the link is
-// optionally reunitarised in FP64.  Uniforms come from the link's own
-// counter-based stream (rng.hpp), 13 centred ones for the proposal and one
-// open one for the test per hit, in that order.
+// from the last accepted hit; accept iff exp(-dS) > u (GLAC/system.cpp:347).
+// After the hits the link is optionally reunitarised in FP64, keeping the
+// sign of det U (three stored rows may hold GLAC's det = -1 links).  Uniforms
+// come from the link's own counter-based stream (rng.hpp): per hit, GLAC's 13
+// centred ones for the proposal as 16-bit halves of seven outputs, then one
+// 32-bit open one for the test.
 #include "kernels.hpp"
 #include "proposal.hpp"
 
@@ -54,7 +56,10 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 2 : 1) void metropolis_ker
         }
         if (prm.reunit) {
             M3<double> d = convert<double>(u);
-            reunitarize<R, sizeof(T) == 4>(d);  // R = 2: the store keeps rows 0 and 1 only
+            if constexpr (R == 2)
+                reunitarize<2, true>(d);  // the store keeps rows 0 and 1 only
+            else
+                reunitarize_keep_det_sign<sizeof(T) == 4>(d);  // det U = +1 or -1 kept
             u = convert<T>(d);
         }
         store_link<T, R>(f, g, mu, idx, u);

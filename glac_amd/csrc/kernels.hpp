@@ -56,7 +56,9 @@ template <int N, int WAVES = kBlock / 64> __device__ __forceinline__ void block_
 // ---- launchers (explicitly instantiated for float and double) ------------
 template <class T, int R> void launch_pack(const double *stage, void *f, const Geom &g, int site_major, hipStream_t s);
 template <class T, int R> void launch_unpack(const void *f, double *stage, const Geom &g, int site_major, hipStream_t s);
-template <class T, int R> void launch_hot_start(void *f, const Geom &g, uint64_t seed, hipStream_t s);
+// Hot-start kinds (the values of the C ABI's GLACX_HOT_*).
+enum HotKind { HOT_HAAR = 0, HOT_GLAC_RANDOM = 1, HOT_RST = 2 };
+template <class T, int R> void launch_hot_start(void *f, const Geom &g, uint64_t seed, int kind, double eps, hipStream_t s);
 template <class T, int R> void launch_cold_start(void *f, const Geom &g, hipStream_t s);
 template <class T, int R>
 void launch_action_derivative(const void *f, double *stage, const Geom &g, int mu, hipStream_t s);

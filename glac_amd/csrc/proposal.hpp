@@ -105,24 +105,20 @@ template <class T, int R> __device__ __forceinline__ M3<T> propose_link(const M3
     }
 }
 
+// GLAC's 13 uniforms in its order (r0..r3, s0..s3, t0..t3, coin) are the 16-bit
+// halves of seven stream outputs, low half first (the 14th half is unused):
+// seven 32-bit draws per proposal instead of thirteen.
 template <class T> __device__ __forceinline__ M3<T> rst_proposal(Stream &rs, T x0abs, T eps)
 {
+    T q[14];
+#pragma unroll
+    for (int k = 0; k < 7; k++) rs.centered_pair<T>(q[2 * k], q[2 * k + 1]);
     cpx<T> ra, rb, sa, sb, ta, tb;
-    {
-        const T q0 = rs.centered<T>(), q1 = rs.centered<T>(), q2 = rs.centered<T>(), q3 = rs.centered<T>();
-        su2_near_unity(q0, q1, q2, q3, x0abs, eps, ra, rb);
-    }
-    {
-        const T q0 = rs.centered<T>(), q1 = rs.centered<T>(), q2 = rs.centered<T>(), q3 = rs.centered<T>();
-        su2_near_unity(q0, q1, q2, q3, x0abs, eps, sa, sb);
-    }
-    {
-        const T q0 = rs.centered<T>(), q1 = rs.centered<T>(), q2 = rs.centered<T>(), q3 = rs.centered<T>();
-        su2_near_unity(q0, q1, q2, q3, x0abs, eps, ta, tb);
-    }
-    const T coin = rs.centered<T>();
+    su2_near_unity(q[0], q[1], q[2], q[3], x0abs, eps, ra, rb);
+    su2_near_unity(q[4], q[5], q[6], q[7], x0abs, eps, sa, sb);
+    su2_near_unity(q[8], q[9], q[10], q[11], x0abs, eps, ta, tb);
     const M3<T> xm = rst_product(ra, rb, sa, sb, ta, tb);
-    return coin < T(0) ? dagger(xm) : xm;
+    return q[12] < T(0) ? dagger(xm) : xm;
 }
 
 } // namespace glacx

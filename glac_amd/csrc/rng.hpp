@@ -62,8 +62,19 @@ struct Stream {
         return result;
     }
 
-    // uniform in [-1/2, 1/2) with 24 bits: exact in float and in double
-    template <class T> __device__ __forceinline__ T centered() { return T(next() >> 8) * T(0x1p-24) - T(0.5); }
+    // two uniforms on the 2^16 midpoints of [-1/2, 1/2) from one 32-bit
+    // output, lo from its low and hi from its high 16 bits:
+    // (k + 1/2) 2^-16 - 1/2 = (2k - 2^16 + 1) 2^-17, exact in float and double
+    // and never zero
+    template <class T> __device__ __forceinline__ void centered_pair(T &lo, T &hi)
+    {
+        const uint32_t b = next();
+        lo = T(b & 0xFFFFu) * T(0x1p-16) - T(0.5 - 0x1p-17);
+        hi = T(b >> 16) * T(0x1p-16) - T(0.5 - 0x1p-17);
+    }
+
+    // uniform in (-1, 1) with 32 bits, in double: (k + 1/2) 2^-31 - 1
+    __device__ __forceinline__ double symmetric() { return ((double)next() + 0.5) * 0x1p-31 - 1.0; }
 
     // uniform in (0, 1) with 32 bits, in double
     __device__ __forceinline__ double open() { return ((double)next() + 0.5) * 0x1p-32; }

@@ -371,4 +371,30 @@ template <int ROWS = 3, bool FAST_NORM = false> __device__ __forceinline__ void 
     }
 }
 
+// Reunitarisation for three stored rows that keeps the sign of det U: rows 0
+// and 1 by Gram-Schmidt, row 2 = s conj(row0 x row1) with s = sign(Re det U)
+// of the input.  det U = sum_j U_2j C_j with the cofactors
+// C_j = U_0k U_1l - U_0l U_1k ((j, k, l) cyclic), and conj(C) is the third
+// row of determinant +1, so s = -1 rebuilds a det = -1 link as det = -1.
+// GLAC's generateRandom hot start makes such links and its SU(3) updates keep
+// them (GLAC/math/matrices/su3matrixgenerator.h:156-161, GLAC/system.cpp:330);
+// for det = +1 links this is reunitarize<3>.
+template <bool FAST_NORM = false> __device__ __forceinline__ void reunitarize_keep_det_sign(M3<double> &m)
+{
+    reunitarize<2, FAST_NORM>(m);  // rows 0 and 1; row 2 still the input's
+    cpx<double> cof[3];
+    double det_re = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int k = (j + 1) % 3, l = (j + 2) % 3;
+        const cpx<double> x = cmul(m.e[0][k], m.e[1][l]);
+        const cpx<double> y = cmul(m.e[0][l], m.e[1][k]);
+        cof[j] = {x.r - y.r, x.i - y.i};
+        det_re += m.e[2][j].r * cof[j].r - m.e[2][j].i * cof[j].i;
+    }
+    const double s = det_re < 0.0 ? -1.0 : 1.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) m.e[2][j] = {s * cof[j].r, -(s * cof[j].i)};
+}
+
 } // namespace glacx

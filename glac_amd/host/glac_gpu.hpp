@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/glacx.h"
@@ -95,6 +96,35 @@ public:
         }
     }
 
+    // Device field that `lattice` is about to be overwritten into on the GPU
+    // (a start or a file load): nothing is uploaded, the device copy is newer.
+    template <class L> int overwrite(L *lattice)
+    {
+        Binding &b = bind(lattice);
+        b.host_newer = false;
+        b.device_newer = true;
+        return b.field;
+    }
+    // Give `lattice` a device field now (its host copy is uploaded on first use).
+    template <class L> void reserve(L *lattice) { bind(lattice); }
+    // dst <- src on the device (System::copyToFlowLattice, GLAC/system.cpp:
+    // 179-188); both lattices must hold device fields of their own.
+    template <class L> void copyField(L *dst, L *src)
+    {
+        const int fs = acquire(src);
+        Binding &bd = bind(dst);
+        if (bd.field == fs || !bound(src)) fatal("copyField: source and destination need device fields of their own");
+        check(glacx_copy_field(m_ctx, bd.field, fs), "copy_field");
+        bd.host_newer = false;
+        bd.device_newer = true;
+    }
+    // Starts of System::latticeSetup (GLAC/system.cpp:200-224) on the device.
+    template <class L> void hotStart(L *lattice, unsigned long long seed, int kind, double eps)
+    {
+        check(glacx_hot_start(m_ctx, overwrite(lattice), seed, kind, eps), "hot_start");
+    }
+    template <class L> void coldStart(L *lattice) { check(glacx_cold_start(m_ctx, overwrite(lattice)), "cold_start"); }
+
     // Read a configuration file straight into the device copy of `lattice`
     // (GLAC's .bin, or Chroma's big-endian variant); the host copy is then
     // stale until syncHost().
@@ -120,6 +150,13 @@ private:
         if (l.m_latticeSize != m_volume || l.m_sites.size() != m_volume) fatal("lattice size does not match the device");
         static_assert(sizeof(l.m_sites[0]) == 18 * sizeof(double), "SU3 must be 18 doubles");
         return reinterpret_cast<double *>(&l.m_sites[0]);
+    }
+
+    template <class L> bool bound(L *lattice) const
+    {
+        for (const Binding &b : m_bind)
+            if (b.host == lattice) return true;
+        return false;
     }
 
     template <class L> Binding &bind(L *lattice)
@@ -356,17 +393,21 @@ public:
         : m_dev(device), m_beta(beta), m_eps(eps), m_nhit(nUpdates), m_seed(seed), m_reunit(reunitEvery)
     {
     }
-    void update(L *lattice)
+    // nSweeps sweeps (GLAC calls update() once per sweep); each sweep's
+    // acceptance rate is added to the score.
+    void update(L *lattice, int nSweeps = 1)
     {
         const int f = m_dev->acquire(lattice);
         if (f != GLACX_FIELD_MAIN) fatal("the Markov chain must live in the main device field");
-        double acc = 0.0;
-        m_dev->check(glacx_metropolis(m_dev->ctx(), 1, m_beta, m_eps, m_nhit, m_reunit, m_seed, m_sweep, &acc, nullptr),
+        std::vector<double> acc((size_t)nSweeps);
+        m_dev->check(glacx_metropolis(m_dev->ctx(), nSweeps, m_beta, m_eps, m_nhit, m_reunit, m_seed, m_sweep,
+                                      acc.data(), nullptr),
                      "metropolis");
-        m_sweep++;
-        m_acceptanceScore += acc;
+        m_sweep += (unsigned long long)nSweeps;
+        for (double a : acc) m_acceptanceScore += a;
         m_dev->markDeviceModified(lattice);
     }
+    void resetAcceptanceScore() { m_acceptanceScore = 0.0; }
     double acceptanceScore() const { return m_acceptanceScore; }
     unsigned long long sweeps() const { return m_sweep; }
 
@@ -377,6 +418,211 @@ private:
     unsigned long long m_seed, m_sweep = 0;
     int m_reunit;
     double m_acceptanceScore = 0.0;
+};
+
+// ------------------------------------------------ GLAC class hierarchy shims ---
+// GLAC's System holds `Action *m_S` and `Correlator *m_correlator,
+// *m_flowCorrelator`, chosen by the "action" and observable strings
+// (GLAC/system.cpp:46-125).  These templates derive from a GLAC class given as
+// `Base` and override its virtuals with the GPU path, so an instance can be
+// stored in those members and called through them.
+
+// Base: GLAC's Action (GLAC/actions/action.h:22-69) or WilsonGaugeAction.
+// computeStaple / getDeltaAction keep GLAC's per-link FP64 semantics on the
+// host (the staple is cached between the two calls, as GLAC's own);
+// getActionDerivative is the GPU flow force.
+template <class Base, class SU3, class L> class GpuAction : public Base {
+public:
+    template <class... A>
+    GpuAction(double beta, Device *device, A &&...baseArgs) : Base(std::forward<A>(baseArgs)...), m_impl(beta, device)
+    {
+    }
+    double getDeltaAction(SU3 &U, SU3 &UPrime) override { return m_impl.getDeltaAction(U, UPrime); }
+    void computeStaple(L *lattice, const int i, const int j, const int k, const int l, const int mu) override
+    {
+        m_impl.computeStaple(lattice, i, j, k, l, mu);
+    }
+    L getActionDerivative(L *lattice, const int mu) override { return m_impl.getActionDerivative(lattice, mu); }
+
+private:
+    WilsonGaugeAction<SU3, L> m_impl;
+};
+
+// Base: GLAC's MasterSamplerTopcXYZ (GLAC/observables/mastersamplertopcxyz.h)
+// or any Correlator whose copyObservable(iObs, obs) stores GLAC's observation
+// vector {plaq, topc, energy, topct[0..Nt-1]} (mastersamplertopcxyz.cpp:
+// 213-235).  calculate() measures on the GPU and hands the vector to the
+// base, so GLAC's own storers, printing, statistics and file writers see it.
+template <class Base, class L> class GpuObservables : public Base {
+public:
+    template <class... A>
+    GpuObservables(Device *device, A &&...baseArgs) : Base(std::forward<A>(baseArgs)...), m_dev(device)
+    {
+    }
+    void calculate(L *lattice, const unsigned int iObs) override
+    {
+        const int f = m_dev->acquire(lattice);
+        std::vector<double> o(3 + (size_t)m_dev->dims()[3]);
+        m_dev->check(glacx_measure(m_dev->ctx(), f, &o[0], &o[2], &o[1], &o[3]), "measure");
+        Base::copyObservable(iObs, o);
+    }
+    // GLAC's System::flowConfiguration loop (GLAC/system.cpp:494-508) in one
+    // call: observations 0..nFlows of the lattice flowed in place, each step's
+    // first RK3 stage measuring the field it reads when the links are FP32.
+    void flowConfiguration(L *lattice, unsigned int nFlows, double epsilon)
+    {
+        const int f = m_dev->acquire(lattice);
+        const int nt = m_dev->dims()[3];
+        const size_t m = nFlows + 1;
+        std::vector<double> p(m), e(m), q(m), qt(m * (size_t)nt);
+        m_dev->check(glacx_flow(m_dev->ctx(), f, (int)nFlows, epsilon, p.data(), e.data(), q.data(), qt.data()),
+                     "flow");
+        m_dev->markDeviceModified(lattice);
+        std::vector<double> o(3 + (size_t)nt);
+        for (size_t i = 0; i < m; i++) {
+            o[0] = p[i];
+            o[1] = q[i];
+            o[2] = e[i];
+            std::copy(qt.begin() + i * nt, qt.begin() + (i + 1) * nt, o.begin() + 3);
+            Base::copyObservable((unsigned int)i, o);
+        }
+    }
+
+private:
+    Device *m_dev;
+};
+
+// --------------------------------------------------------------- run loop ---
+// The keys of GLAC's JSON configuration the path consumes
+// (GLAC/config/configloader.cpp:55-181, defaults GLAC/config/parameters.cpp:
+// 5-93).
+struct RunParameters {
+    std::vector<unsigned int> N{8, 8, 8, 16};  // NSpatial x 3, NTemporal
+    double beta = 6.0, SU3Eps = 0.24, flowEpsilon = 0.01;
+    unsigned int NTherm = 10, NCf = 30, NCor = 10, NUpdates = 10, NFlows = 0;
+    bool hotStart = true, RSTHotStart = false;
+    bool storeThermalizationObservables = false;
+    unsigned long long metropolisSeed = 1, randomMatrixSeed = 2;
+    int precision = GLACX_FP64;
+    int reunitEvery = 1;      // FP64 reunitarisation every this many sweeps (0: never, as GLAC)
+    int device = 0;
+    std::string configPrefix;  // non-empty: write every configuration as <prefix>_config%05d.bin
+    bool verbose = false;
+};
+
+// GLAC's System (GLAC/system.h:117-121): latticeSetup() then run(), i.e.
+// thermalize + runMetropolis with a flowConfiguration per configuration
+// (GLAC/system.cpp:193-231, 262-319, 363-517), on the device.  Corr and
+// FlowCorr are GLAC Correlator types (typically GpuObservables over GLAC's
+// MasterSamplerTopcXYZ, measuring on the same Device); the flow correlator
+// also needs flowConfiguration().  The Device (built from p.N, p.precision,
+// p.device) holds the chain lattice in its main field and the flow lattice in
+// the other, GLAC's Lattice<SU3>[4] each; syncHost() brings the chain to the
+// host.
+template <class L, class Corr, class FlowCorr> class System {
+public:
+    System(Device *device, const RunParameters &p, Corr *correlator, FlowCorr *flowCorrelator)
+        : m_p(p), m_dev(*device), m_correlator(correlator), m_flowCorrelator(flowCorrelator),
+          m_update(device, p.beta, p.SU3Eps, (int)p.NUpdates, p.metropolisSeed, p.reunitEvery)
+    {
+        for (int k = 0; k < 4; k++)
+            if ((int)p.N.at(k) != device->dims()[k]) fatal("System: lattice extents differ from the device's");
+        for (int mu = 0; mu < 4; mu++) {
+            m_lattice[mu] = L(p.N);
+            m_flowLattice[mu] = L(p.N);
+        }
+    }
+
+    Device &device() { return m_dev; }
+    L *lattice() { return m_lattice; }
+    L *flowLattice() { return m_flowLattice; }
+    double getAcceptanceRate() const { return m_acceptanceRate; }
+
+    // Hot start (generateRST if RSTHotStart, else GLAC's generateRandom,
+    // det = -1) or cold start, GLAC/system.cpp:200-224, from the device's
+    // link streams of randomMatrixSeed.  The chain lattice takes the main
+    // device field, the flow lattice the other one.
+    void latticeSetup()
+    {
+        if (m_p.hotStart)
+            m_dev.hotStart(m_lattice, m_p.randomMatrixSeed, m_p.RSTHotStart ? GLACX_HOT_RST : GLACX_HOT_GLAC_RANDOM,
+                           m_p.SU3Eps);
+        else
+            m_dev.coldStart(m_lattice);
+        m_dev.reserve(m_flowLattice);
+    }
+
+    void run()
+    {
+        thermalize();
+        runMetropolis();
+    }
+
+    void thermalize()
+    {
+        m_NThermSteps = m_p.storeThermalizationObservables ? 1 + m_p.NTherm : 0;
+        if (m_p.storeThermalizationObservables) {
+            m_correlator->calculate(m_lattice, 0);
+            for (unsigned int i = 1; i <= m_p.NTherm; i++) {
+                m_update.update(m_lattice);
+                m_correlator->calculate(m_lattice, i);
+            }
+        } else if (m_p.NTherm > 0) {
+            m_update.update(m_lattice, (int)m_p.NTherm);
+        }
+        if (m_p.verbose && m_p.NTherm)
+            std::printf("Termalization complete. Acceptance rate: %f\n", m_update.acceptanceScore() / m_p.NTherm);
+    }
+
+    void runMetropolis()
+    {
+        m_update.resetAcceptanceScore();
+        for (unsigned int iConfig = 0; iConfig < m_p.NCf; iConfig++) {
+            m_update.update(m_lattice, (int)m_p.NCor);
+            if (m_p.NFlows != 0) {
+                m_dev.copyField(m_flowLattice, m_lattice);
+                flowConfiguration(iConfig);
+                m_correlator->copyObservable(iConfig + m_NThermSteps, m_flowCorrelator->getObservablesVector(0));
+            } else {
+                m_correlator->calculate(m_lattice, iConfig + m_NThermSteps);
+            }
+            if (m_p.verbose) {
+                std::printf("%-4u ", iConfig);
+                m_correlator->printObservable(iConfig + m_NThermSteps);
+                std::printf("\n");
+            }
+            if (!m_p.configPrefix.empty()) {
+                char name[32];
+                std::snprintf(name, sizeof name, "_config%05u.bin", iConfig);
+                m_dev.check(glacx_save_bin(m_dev.ctx(), m_dev.acquire(m_lattice), (m_p.configPrefix + name).c_str()),
+                            "save_bin");
+            }
+        }
+        m_acceptanceRate = m_p.NCf ? m_update.acceptanceScore() / double(m_p.NCf * m_p.NCor) : 0.0;
+        m_correlator->runStatistics();
+        m_correlator->writeObservableToFile(m_acceptanceRate);
+    }
+
+    // System::flowConfiguration (GLAC/system.cpp:471-517): reset, observations
+    // 0..NFlows of the flowed copy, then the flow files of configuration iConfig.
+    void flowConfiguration(unsigned int iConfig)
+    {
+        m_flowCorrelator->reset();
+        m_flowCorrelator->flowConfiguration(m_flowLattice, m_p.NFlows, m_p.flowEpsilon);
+        m_flowCorrelator->writeFlowObservablesToFile(iConfig);
+    }
+
+    void syncHost() { m_dev.syncHost(m_lattice); }
+
+private:
+    RunParameters m_p;
+    Device &m_dev;
+    Corr *m_correlator;
+    FlowCorr *m_flowCorrelator;
+    Metropolis<L> m_update;
+    L m_lattice[4], m_flowLattice[4];
+    unsigned int m_NThermSteps = 0;
+    double m_acceptanceRate = 0.0;
 };
 
 } // namespace glac_gpu

@@ -47,15 +47,24 @@ def train_from_chain(L, beta: float, eps: float = 0.24, n_hit: int = 10, collect
     """Collect about `samples` (staple, link) pairs from `collect_sweeps`
     RST sweeps of the chain field (continuing it from `sweep_offset`), set
     the feature standardisation, initialise the network and train it for
-    `steps` Adam steps.  Returns the per-step loss (negative ESJD).  The
+    `steps` Adam steps.  Returns the per-step loss (negative ESJD) as an
+    array whose attribute `samples` is the number of training samples.  The
     chain field ends `collect_sweeps * sweeps_between` sweeps further on."""
     vh = L.V // 2
     stride = max(1, (4 * L.V * collect_sweeps) // samples)     # 8 (mu, parity) passes of V/2 links
     L.samples_reset(8 * ((vh + stride - 1) // stride) * collect_sweeps)
+    count = 0
     for i in range(collect_sweeps):
         L.metropolis(sweeps_between, beta, eps, n_hit, seed=seed,
                      sweep_offset=sweep_offset + i * sweeps_between)
-        L.samples_collect(stride=stride)
+        count = L.samples_collect(stride=stride)
     mean, std = L.samples_stats(beta)
     L.proposal_set(init_params(init_seed), mean, std)
-    return L.train(steps, batch=batch, draws=draws, lr=lr, beta=beta, seed=seed)
+    loss = L.train(steps, batch=batch, draws=draws, lr=lr, beta=beta, seed=seed).view(TrainLoss)
+    loss.samples = count
+    return loss
+
+
+class TrainLoss(np.ndarray):
+    """Per-step training loss; `samples` holds the training-set size."""
+    samples = 0

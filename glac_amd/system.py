@@ -34,6 +34,7 @@ class EnsembleConfig:
     seed: int = 1
     proposal: str = "rst"          # "rst" (GLAC) or "learned"
     reunit_every: int = 1
+    sweep_plaquette: bool = False  # also record the chain plaquette after every sweep
 
 
 @dataclass
@@ -42,6 +43,8 @@ class EnsembleResult:
     q: np.ndarray = field(default_factory=lambda: np.empty(0))         # Q at flow time NFlows * eps
     e: np.ndarray = field(default_factory=lambda: np.empty(0))         # E = -E_raw / 64 at that time
     acceptance: float = 0.0
+    sweep_plaq: np.ndarray = field(default_factory=lambda: np.empty(0))  # per sweep (sweep_plaquette)
+    sweep_acc: np.ndarray = field(default_factory=lambda: np.empty(0))   # acceptance per sweep
     update_seconds: float = 0.0
     flow_seconds: float = 0.0
     therm_seconds: float = 0.0
@@ -78,11 +81,14 @@ def run_ensemble(L: Lattice, cfg: EnsembleConfig, thermalise=True, sweep_offset=
         L.synchronize()
         res.therm_seconds = time.perf_counter() - t0
         sweep += cfg.NTherm
-    plaq, q, e, acc = [], [], [], []
+    plaq, q, e, acc, splaq = [], [], [], [], []
     for i in range(cfg.NCf):
         t0 = time.perf_counter()
         a = update(cfg.NCor, cfg.beta, cfg.SU3Eps, cfg.NUpdates, reunit_every=cfg.reunit_every,
-                   seed=cfg.seed, sweep_offset=sweep)
+                   seed=cfg.seed, sweep_offset=sweep, plaquette=cfg.sweep_plaquette)
+        if cfg.sweep_plaquette:
+            a, p_sweeps = a
+            splaq.append(p_sweeps)
         L.synchronize()
         t1 = time.perf_counter()
         sweep += cfg.NCor
@@ -96,10 +102,12 @@ def run_ensemble(L: Lattice, cfg: EnsembleConfig, thermalise=True, sweep_offset=
         plaq.append(p_chain)
         q.append(qq)
         e.append(-e_raw / 64.0)
-        acc.append(a.mean())
+        acc.append(a)
         if progress is not None:
             progress(i, qq)
     res.plaq, res.q, res.e = np.array(plaq), np.array(q), np.array(e)
-    res.acceptance = float(np.mean(acc)) if acc else 0.0
+    res.sweep_acc = np.concatenate(acc) if acc else np.empty(0)
+    res.sweep_plaq = np.concatenate(splaq) if splaq else np.empty(0)
+    res.acceptance = float(res.sweep_acc.mean()) if acc else 0.0
     res.sweeps_done = sweep
     return res

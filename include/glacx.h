@@ -91,10 +91,22 @@ int glacx_save_bin(glacx_ctx *ctx, int field, const char *path);
  * double big-endian. */
 int glacx_load_chroma(glacx_ctx *ctx, int field, const char *path);
 
-/* Starts (GLAC/system.cpp:200-224).  Hot start draws Haar-random SU(3) links
- * from the counter-based stream (seed, site, mu); cold start sets unity. */
+/* Starts (System::latticeSetup, GLAC/system.cpp:200-224).  Cold start sets
+ * every link to unity.  Hot start draws every link from its own
+ * counter-based stream (seed, site, mu), of one of three kinds:
+ *   GLACX_HOT_HAAR         Haar-random SU(3) (eps unused);
+ *   GLACX_HOT_GLAC_RANDOM  GLAC's default "fully random" start,
+ *                          SU3MatrixGenerator::generateRandom
+ *                          (GLAC/math/matrices/su3matrixgenerator.h:98-164):
+ *                          entries uniform in (-eps, eps), columns
+ *                          orthonormalised, det U = -1.  Not available with
+ *                          GLACX_FP32_COMPRESSED (returns GLACX_E_ARG);
+ *   GLACX_HOT_RST          GLAC's RSTHotStart, generateRST with step eps
+ *                          (su3matrixgenerator.h:170-196): SU(3) near unity.
+ * eps must lie in (0, 1) for the last two. */
+enum glacx_hot_kind { GLACX_HOT_HAAR = 0, GLACX_HOT_GLAC_RANDOM = 1, GLACX_HOT_RST = 2 };
 int glacx_cold_start(glacx_ctx *ctx, int field);
-int glacx_hot_start(glacx_ctx *ctx, int field, uint64_t seed);
+int glacx_hot_start(glacx_ctx *ctx, int field, uint64_t seed, int kind, double eps);
 
 /* n_sweeps Metropolis sweeps on GLACX_FIELD_MAIN (System::update,
  * GLAC/system.cpp:333-361, with WilsonGaugeAction::computeStaple /

@@ -39,6 +39,7 @@ def lib():
         if not os.path.exists(_LIB_PATH):
             build()
         L = C.CDLL(_LIB_PATH)
+        L.or_num_threads.restype = C.c_int
         L.or_su3_mul.argtypes = [_dp, _dp, _dp]
         L.or_su3_dag.argtypes = [_dp, _dp]
         L.or_su3_retrace_mul.argtypes = [_dp, _dp]
@@ -74,7 +75,9 @@ def lib():
         L.or_glac_sweep.argtypes = [_dp, _i4, C.c_void_p, C.c_double, C.c_double, C.c_int]
         L.or_glac_sweep.restype = C.c_double
         L.or_reunitarize.argtypes = [_dp]
+        L.or_reunitarize_keep_det_sign.argtypes = [_dp]
         L.or_hot_start_haar.argtypes = [_dp, _i4, C.c_uint64]
+        L.or_hot_start.argtypes = [_dp, _i4, C.c_uint64, C.c_int, C.c_double]
         L.or_cb_sweep.argtypes = [_dp, _i4, C.c_double, C.c_double, C.c_int, C.c_uint64,
                                   C.c_uint64, C.c_int]
         L.or_cb_sweep.restype = C.c_double
@@ -88,6 +91,11 @@ def lib():
         L.or_learned_cb_sweep.restype = C.c_double
         _lib = L
     return _lib
+
+
+def num_threads() -> int:
+    """Threads of the oracle's parallel site loops (OpenMP)."""
+    return int(lib().or_num_threads())
 
 
 # ---------------------------------------------------------------- conversions
@@ -291,6 +299,23 @@ def reunitarize(m) -> np.ndarray:
     m = np.array(m, dtype=np.float64)
     lib().or_reunitarize(m)
     return m
+
+
+def reunitarize_keep_det_sign(m) -> np.ndarray:
+    m = np.array(m, dtype=np.float64)
+    lib().or_reunitarize_keep_det_sign(m)
+    return m
+
+
+HOT_HAAR, HOT_GLAC_RANDOM, HOT_RST = 0, 1, 2
+
+
+def hot_start(n, seed, kind=HOT_HAAR, eps=0.24) -> np.ndarray:
+    """The device's hot starts on the same link streams: Haar SU(3), GLAC's
+    generateRandom (det = -1) or GLAC's RST matrix with step eps."""
+    U = new_field(n)
+    lib().or_hot_start(U, dims4(n), seed, kind, eps)
+    return U
 
 
 def hot_start_haar(n, seed) -> np.ndarray:

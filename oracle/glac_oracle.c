@@ -68,19 +68,39 @@
  *      near GLAC's recorded 0.108 at 4^3x8, beta 6, eps 0.24, 10 hits
  *      (examples/test_config/configExampleLoad.out:72).  GLAC's exact chain is
  *      parity unpinned (GLAC needs MPI and is not built here).
- *  or_reunitarize / or_hot_start_haar / or_cb_sweep   no GLAC counterpart.
+ *  or_reunitarize / or_reunitarize_keep_det_sign / or_hot_start_haar /
+ *  or_cb_sweep   no GLAC counterpart.
  *      Pinned: unitarity and det = 1; Haar moments <|tr U|^2> = 1;
  *      single-link histogram against exp(beta/3 Re tr UA); same plaquette
- *      distribution as or_glac_sweep.
+ *      distribution as or_glac_sweep; GLAC's recorded acceptance and
+ *      plaquette (configExampleLoad.out:58-73); det = -1 kept by
+ *      or_reunitarize_keep_det_sign.
+ *  or_hot_start   GLAC/system.cpp:200-214 on the link streams.  Pinned: the
+ *      GLAC_RANDOM and RST kinds have the moments of GLAC's own generators
+ *      on its mt19937_64 stream (or_glac_hot_start), det = -1 / +1,
+ *      unitarity; the HAAR kind is or_hot_start_haar.
  */
 #include "glac_oracle.h"
 
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 
 #define RE(m, i, j) ((m)[6 * (i) + 2 * (j)])
 #define IM(m, i, j) ((m)[6 * (i) + 2 * (j) + 1])
+
+/* Threads the parallel site loops use (OpenMP; 1 in a build without it). */
+int or_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 /* ======================================================================== */
 /* 3x3 complex algebra                                                       */
@@ -245,15 +265,30 @@ static void link_stream(uint32_t st[4], uint64_t seed, long site, int mu, int ta
     or_philox4x32_10(ctr, key, st);
     if ((st[0] | st[1] | st[2] | st[3]) == 0u) st[0] = 1u;
 }
-/* uniform in [-1/2, 1/2) with 24 bits, exact in float and double */
-static inline double stream_centered(uint32_t s[4])
-{
-    return (double)(or_xoshiro128ss_next(s) >> 8) * 0x1p-24 - 0.5;
-}
 /* uniform in (0, 1) with 32 bits */
 static inline double stream_open(uint32_t s[4])
 {
     return ((double)or_xoshiro128ss_next(s) + 0.5) * 0x1p-32;
+}
+/* two uniforms on the 2^16 midpoints of [-1/2, 1/2) from one output: lo from
+ * its low, hi from its high 16 bits */
+static inline void stream_centered_pair(uint32_t s[4], double *lo, double *hi)
+{
+    const uint32_t b = or_xoshiro128ss_next(s);
+    *lo = (double)(b & 0xFFFFu) * 0x1p-16 - (0.5 - 0x1p-17);
+    *hi = (double)(b >> 16) * 0x1p-16 - (0.5 - 0x1p-17);
+}
+/* GLAC's 13 RST uniforms (r, s, t, coin) as the 16-bit halves of seven outputs */
+static inline void stream_rst_uniforms(uint32_t s[4], double u[13])
+{
+    double q[14];
+    for (int k = 0; k < 7; k++) stream_centered_pair(s, &q[2 * k], &q[2 * k + 1]);
+    memcpy(u, q, 13 * sizeof(double));
+}
+/* uniform in (-1, 1) with 32 bits */
+static inline double stream_symmetric(uint32_t s[4])
+{
+    return ((double)or_xoshiro128ss_next(s) + 0.5) * 0x1p-31 - 1.0;
 }
 
 /* ======================================================================== */
@@ -450,6 +485,7 @@ static void force_from_omega(const double *O, double *Z)
 void or_action_derivative(const double *U, const int n[4], int mu, double *Z)
 {
     long V = volume(n);
+#pragma omp parallel for schedule(static)
     for (long s = 0; s < V; s++) {
         int c[4];
         double A[18], O[18];
@@ -553,6 +589,7 @@ void or_su3_exp(const double *Q, double *out)
 
 static void exp_times_field(double *U, long V, const double *Y, double scale)
 {
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < 4 * V; i++) {
         double Q[18], X[18];
         for (int k = 0; k < 18; k++) Q[k] = scale * Y[i * 18 + k];
@@ -571,10 +608,12 @@ void or_flow_step(double *U, const int n[4], double eps)
     exp_times_field(U, V, Z0, 0.25 * eps);
     /* Y = 8/9 eps Z(W1) - 17/36 eps Z0 ;  W2 = exp(Y) W1 */
     for (int mu = 0; mu < 4; mu++) or_action_derivative(U, n, mu, Z + (long)mu * V * 18);
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < 4 * V * 18; i++) Z0[i] = (8.0 / 9.0) * eps * Z[i] - (17.0 / 36.0) * eps * Z0[i];
     exp_times_field(U, V, Z0, 1.0);
     /* Y' = 3/4 eps Z(W2) - Y ;  W3 = exp(Y') W2 */
     for (int mu = 0; mu < 4; mu++) or_action_derivative(U, n, mu, Z + (long)mu * V * 18);
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < 4 * V * 18; i++) Z0[i] = 0.75 * eps * Z[i] - Z0[i];
     exp_times_field(U, V, Z0, 1.0);
     free(Z0);
@@ -642,24 +681,37 @@ static void traceless_antiherm(double *C)
 void or_measure(const double *U, const int n[4], double out[3], double *qt)
 {
     long V = volume(n);
-    double plaq = 0.0, energy = 0.0;
-    for (int t = 0; t < n[3]; t++) qt[t] = 0.0;
+    /* per-site terms in parallel (10 per site: 6 plaquettes, 3 energy pairs,
+     * the charge), then summed serially in site order, so the result does
+     * not depend on the thread count */
+    double *terms = (double *)malloc((size_t)V * 10 * sizeof(double));
+#pragma omp parallel for schedule(static)
     for (long s = 0; s < V; s++) {
         int c[4];
         decode(n, s, c);
+        double *ts = terms + s * 10;
         double qs = 0.0;
         for (int nu = 1; nu < 4; nu++) {
             int rho = nu % 3 + 1, sigma = rho % 3 + 1;
             double C1[18], C2[18];
-            plaq += clover(U, n, c, 0, nu, C1);
-            plaq += clover(U, n, c, rho, sigma, C2);
+            ts[2 * (nu - 1)] = clover(U, n, c, 0, nu, C1);
+            ts[2 * (nu - 1) + 1] = clover(U, n, c, rho, sigma, C2);
             traceless_antiherm(C1);
             traceless_antiherm(C2);
             qs -= or_su3_retrace_mul(C1, C2);
-            energy += or_su3_retrace_mul(C1, C1) + or_su3_retrace_mul(C2, C2);
+            ts[6 + nu - 1] = or_su3_retrace_mul(C1, C1) + or_su3_retrace_mul(C2, C2);
         }
-        qt[c[3]] += qs;
+        ts[9] = qs;
     }
+    double plaq = 0.0, energy = 0.0;
+    for (int t = 0; t < n[3]; t++) qt[t] = 0.0;
+    for (long s = 0; s < V; s++) {
+        const double *ts = terms + s * 10;
+        for (int k = 0; k < 6; k++) plaq += ts[k];
+        for (int k = 6; k < 9; k++) energy += ts[k];
+        qt[(int)(s / ((long)n[0] * n[1] * n[2]))] += ts[9];
+    }
+    free(terms);
     const double qnorm = 1.0 / (256.0 * M_PI * M_PI);
     double q = 0.0;
     for (int t = 0; t < n[3]; t++) {
@@ -679,6 +731,7 @@ void or_measure_density(const double *U, const int n[4], double *qd, double *ed)
 {
     long V = volume(n);
     const double qnorm = 1.0 / (256.0 * M_PI * M_PI);
+#pragma omp parallel for schedule(static)
     for (long s = 0; s < V; s++) {
         int c[4];
         decode(n, s, c);
@@ -855,9 +908,32 @@ void or_reunitarize(double *m)
     }
 }
 
-/* Haar-random SU(3): 9 complex Gaussians (Box-Muller on 18 stream uniforms,
- * entry k from uniforms 2k, 2k+1), then or_reunitarize. */
-void or_hot_start_haar(double *U, const int n[4], uint64_t seed)
+/* Reunitarisation of a Metropolis-updated link that keeps the sign of det U:
+ * rows 0 and 1 as or_reunitarize, row 2 = s conj(row0 x row1) with
+ * s = sign(Re det U) of the input, det U = sum_j U_2j C_j (C_j the cofactors
+ * of row 2, conj(C) the det = +1 third row).  GLAC's generateRandom start
+ * gives det = -1 links (GLAC/math/matrices/su3matrixgenerator.h:156-161) and
+ * its SU(3) updates keep them; for det = +1 this is or_reunitarize. */
+void or_reunitarize_keep_det_sign(double *m)
+{
+    double row2[6];
+    memcpy(row2, m + 12, sizeof row2);
+    or_reunitarize(m);
+    double det_re = 0.0;
+    for (int j = 0; j < 3; j++) det_re += row2[2 * j] * RE(m, 2, j) + row2[2 * j + 1] * IM(m, 2, j);
+    if (det_re < 0.0)
+        for (int j = 0; j < 3; j++) { RE(m, 2, j) = -RE(m, 2, j); IM(m, 2, j) = -IM(m, 2, j); }
+}
+
+/* Hot starts from the link streams (seed, site, mu, tag 2, sweep 0):
+ *  OR_HOT_HAAR: Haar-random SU(3), 9 complex Gaussians (Box-Muller on 18
+ *      stream uniforms, entry k from uniforms 2k, 2k+1), then or_reunitarize;
+ *  OR_HOT_GLAC_RANDOM: GLAC's generateRandom (su3matrixgenerator.h:98-164) on
+ *      18 uniforms eps * stream_symmetric, entry k from uniforms 2k (re),
+ *      2k+1 (im): det = -1;
+ *  OR_HOT_RST: GLAC's generateRST (su3matrixgenerator.h:170-196) with step
+ *      eps on the 13 uniforms of stream_rst_uniforms (GLAC/system.cpp:208). */
+void or_hot_start(double *U, const int n[4], uint64_t seed, int kind, double eps)
 {
     long V = volume(n);
     for (int mu = 0; mu < 4; mu++)
@@ -865,19 +941,32 @@ void or_hot_start_haar(double *U, const int n[4], uint64_t seed)
             uint32_t st[4];
             link_stream(st, seed, s, mu, STREAM_HOT_START, 0);
             double *m = U + ((long)mu * V + s) * 18;
-            for (int k = 0; k < 9; k++) {
-                double u1 = stream_open(st), u2 = stream_open(st);
-                double r = sqrt(-2.0 * log(u1));
-                m[2 * k] = r * cos(2.0 * M_PI * u2);
-                m[2 * k + 1] = r * sin(2.0 * M_PI * u2);
+            if (kind == OR_HOT_GLAC_RANDOM) {
+                double u[18];
+                for (int k = 0; k < 18; k++) u[k] = eps * stream_symmetric(st);
+                or_generate_random(u, m);
+            } else if (kind == OR_HOT_RST) {
+                double u[13];
+                stream_rst_uniforms(st, u);
+                or_generate_rst(u, eps, m);
+            } else {
+                for (int k = 0; k < 9; k++) {
+                    double u1 = stream_open(st), u2 = stream_open(st);
+                    double r = sqrt(-2.0 * log(u1));
+                    m[2 * k] = r * cos(2.0 * M_PI * u2);
+                    m[2 * k + 1] = r * sin(2.0 * M_PI * u2);
+                }
+                or_reunitarize(m);
             }
-            or_reunitarize(m);
         }
 }
 
+void or_hot_start_haar(double *U, const int n[4], uint64_t seed) { or_hot_start(U, n, seed, OR_HOT_HAAR, 0.0); }
+
 /* One sweep = 8 sub-sweeps (mu = 0..3, parity 0 then 1); a link at site s
- * (parity = x+y+z+t mod 2) draws, per hit, 13 centred uniforms for GLAC's RST
- * proposal and one open uniform for the accept test, from its own stream. */
+ * (parity = x+y+z+t mod 2) draws, per hit, GLAC's 13 centred uniforms for its
+ * RST proposal (16-bit halves of seven outputs, stream_rst_uniforms) and one
+ * open uniform for the accept test, from its own stream. */
 double or_cb_sweep(double *U, const int n[4], double beta, double eps, int nhit,
                    uint64_t seed, uint64_t sweep, int reunit)
 {
@@ -895,7 +984,7 @@ double or_cb_sweep(double *U, const int n[4], double beta, double eps, int nhit,
                 double *Ul = U + ((long)mu * V + s) * 18;
                 staple_at(U, n, c, mu, A);
                 for (int h = 0; h < nhit; h++) {
-                    for (int k = 0; k < 13; k++) u[k] = stream_centered(st);
+                    stream_rst_uniforms(st, u);
                     or_generate_rst(u, eps, X);
                     or_su3_mul(X, Ul, Up);
                     double dS = or_delta_action(Ul, Up, A, beta);
@@ -904,7 +993,7 @@ double or_cb_sweep(double *U, const int n[4], double beta, double eps, int nhit,
                         accepted++;
                     }
                 }
-                if (reunit) or_reunitarize(Ul);
+                if (reunit) or_reunitarize_keep_det_sign(Ul);
             }
     return (double)accepted / (4.0 * nhit * (double)V);
 }
@@ -1067,7 +1156,7 @@ double or_learned_cb_sweep(double *U, const int n[4], double beta, double eps, i
                         or_su3_exp(Y, X);
                     } else {
                         double u[13];
-                        for (int k = 0; k < 13; k++) u[k] = stream_centered(st);
+                        stream_rst_uniforms(st, u);
                         or_generate_rst(u, eps, X);
                     }
                     or_su3_mul(X, Ul, Up);
@@ -1077,7 +1166,7 @@ double or_learned_cb_sweep(double *U, const int n[4], double beta, double eps, i
                         accepted++;
                     }
                 }
-                if (reunit) or_reunitarize(Ul);
+                if (reunit) or_reunitarize_keep_det_sign(Ul);
             }
     return (double)accepted / (4.0 * nhit * (double)V);
 }

@@ -27,6 +27,10 @@
 extern "C" {
 #endif
 
+/* Threads the parallel site loops (flow, force, measurement) use; results do
+ * not depend on it. */
+int or_num_threads(void);
+
 /* ---- 3x3 complex algebra ------------------------------------------------ */
 void   or_su3_mul(const double *a, const double *b, double *c);
 void   or_su3_dag(const double *a, double *c);
@@ -72,6 +76,9 @@ double or_glac_sweep(double *U, const int n[4], or_glac_rng *g, double beta, dou
 
 /* ---- Metropolis: checkerboard sweep with the counter-based link streams -- */
 void   or_reunitarize(double *m);
+void   or_reunitarize_keep_det_sign(double *m);
+enum { OR_HOT_HAAR = 0, OR_HOT_GLAC_RANDOM = 1, OR_HOT_RST = 2 };
+void   or_hot_start(double *U, const int n[4], uint64_t seed, int kind, double eps);
 void   or_hot_start_haar(double *U, const int n[4], uint64_t seed);
 double or_cb_sweep(double *U, const int n[4], double beta, double eps, int nhit,
                    uint64_t seed, uint64_t sweep, int reunit);

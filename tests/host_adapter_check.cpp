@@ -2,15 +2,18 @@
 // stand-in of GLAC's public Lattice<SU3> shape (GLAC/math/lattice.h:23-29).
 //
 //   host_adapter_check cpu   GLAC's dS = -36 test on a unit field
-//                            (GLAC/tests/unit_tests/actiontests.cpp:67-114)
-//   host_adapter_check gpu   flowConfiguration-style loop (calculate, then
-//                            flowField + calculate per step), the same loop
-//                            as one flowConfiguration call (FP64, FP32) and Metropolis
-//                            updates through the adapter, checked against
-//                            the FP64 oracle; device residency is checked by
-//                            counting host<->device transfers.
-//   host_adapter_check gpu <dir>   also writes/reads configurations in <dir>
-//                            through FieldIO (.bin and Chroma).
+//                            (GLAC/tests/unit_tests/actiontests.cpp:67-114),
+//                            directly and through GLAC's Action interface
+//   host_adapter_check gpu [<dir>]   flowConfiguration-style loop (calculate,
+//                            then flowField + calculate per step), the same
+//                            loop as one flowConfiguration call (FP64, FP32)
+//                            and Metropolis updates through the adapter,
+//                            checked against the FP64 oracle, with the host
+//                            copy left stale until syncHost; configurations
+//                            written and read in <dir> through FieldIO (.bin
+//                            and Chroma); then GLAC's System run loop
+//                            (glac_gpu::System with GLAC-interface
+//                            correlators) against the oracle's chain.
 // Prints "PASS ..." lines; exits non-zero on the first failure.
 #include <cmath>
 #include <cstdio>
@@ -37,6 +40,62 @@ template <class T> struct Lattice {
         m_latticeSize = (unsigned long)dim[0] * dim[1] * dim[2] * dim[3];
         m_sites.assign(m_latticeSize, T());
     }
+};
+
+// Stand-ins of GLAC's abstract bases with the signatures of
+// GLAC/actions/action.h:49-68 and GLAC/observables/correlator.h:46-67 (GLAC's
+// own headers need MPI), and of GLAC's MasterSamplerTopcXYZ storage
+// (GLAC/observables/mastersamplertopcxyz.cpp:213-235): per observation the
+// vector {plaq, topc, energy, topct[0..Nt-1]}.  The flow-file writer records
+// the observations it would write.
+class Action {
+public:
+    virtual ~Action() {}
+    virtual double getDeltaAction(SU3 &U, SU3 &UPrime) = 0;
+    virtual void computeStaple(Lattice<SU3> *lattice, const int i, const int j, const int k, const int l,
+                               const int mu) = 0;
+    virtual Lattice<SU3> getActionDerivative(Lattice<SU3> *lattice, const int mu) = 0;
+};
+
+class Correlator {
+public:
+    explicit Correlator(bool storeFlowObservable) : m_storeFlowObservable(storeFlowObservable) {}
+    virtual ~Correlator() {}
+    virtual void calculate(Lattice<SU3> *lattice, unsigned int iObs) = 0;
+    virtual void writeObservableToFile(const double acceptanceRatio) { m_acceptanceWritten = acceptanceRatio; }
+    virtual void writeFlowObservablesToFile(const unsigned int) {}
+    virtual void runStatistics() {}
+    virtual void printObservable(const unsigned int) {}
+    virtual void printHeader() {}
+    virtual void printStatistics() {}
+    virtual void copyObservable(const unsigned int iObs, const std::vector<double> &obs) = 0;
+    virtual std::vector<double> getObservablesVector(const unsigned int iObs) = 0;
+    virtual double getObservable(const unsigned int iObs) { return getObservablesVector(iObs)[0]; }
+    virtual void reset() {}
+    bool m_storeFlowObservable;
+    double m_acceptanceWritten = -1.0;
+};
+
+class MasterSamplerTopcXYZ : public Correlator {
+public:
+    MasterSamplerTopcXYZ(bool flow, unsigned int nObs, unsigned int nt)
+        : Correlator(flow), m_obs(nObs, std::vector<double>(3 + nt, 0.0))
+    {
+    }
+    void calculate(Lattice<SU3> *, unsigned int) override { std::abort(); }  // GLAC's CPU clover: not used here
+    void copyObservable(const unsigned int iObs, const std::vector<double> &obs) override { m_obs.at(iObs) = obs; }
+    std::vector<double> getObservablesVector(const unsigned int iObs) override { return m_obs.at(iObs); }
+    void reset() override
+    {
+        for (auto &o : m_obs) std::fill(o.begin(), o.end(), 0.0);
+    }
+    void writeFlowObservablesToFile(const unsigned int iConfig) override
+    {
+        m_flowFiles.resize(iConfig + 1);
+        m_flowFiles[iConfig] = m_obs;
+    }
+    std::vector<std::vector<double>> m_obs;
+    std::vector<std::vector<std::vector<double>>> m_flowFiles;
 };
 
 #define REQUIRE(cond, ...)                                                                       \
@@ -76,6 +135,101 @@ static int run_cpu()
                 REQUIRE(std::fabs(dS + 36.0) <= 1e-15, "dS = %.17g", dS);
             }
     std::printf("PASS dS = -36 on the unit field\n");
+    // the same through GLAC's Action interface (GpuAction over the stand-in base)
+    Action *A = new glac_gpu::GpuAction<Action, SU3, Lattice<SU3>>(6.0, nullptr);
+    SU3 &u = L[2].m_sites[5];
+    A->computeStaple(L, 1, 1, 0, 0, 2);
+    const double dS = A->getDeltaAction(u, two);
+    delete A;
+    REQUIRE(std::fabs(dS + 36.0) <= 1e-15, "dS through Action* = %.17g", dS);
+    std::printf("PASS dS = -36 through GLAC's Action interface\n");
+    return 0;
+}
+
+// GLAC's System loop on the device (glac_gpu::System with GLAC-interface
+// correlators), against the oracle running the same chain: GLAC's random hot
+// start, thermalisation and NCf x NCor sweeps on the same link streams, each
+// configuration flowed NFlows steps and measured.
+static int run_system(const std::string &dir)
+{
+    const std::vector<unsigned int> N = {4, 4, 4, 8};
+    const int n[4] = {4, 4, 4, 8};
+    const unsigned long V = 4 * 4 * 4 * 8;
+    glac_gpu::RunParameters p;
+    p.N = N;
+    p.beta = 6.0;
+    p.NTherm = 3;
+    p.NCf = 3;
+    p.NCor = 2;
+    p.NFlows = 5;
+    p.flowEpsilon = 0.02;
+    p.metropolisSeed = 31;
+    p.randomMatrixSeed = 41;
+    p.configPrefix = dir + "/system";
+    using Obs = glac_gpu::GpuObservables<MasterSamplerTopcXYZ, Lattice<SU3>>;
+    glac_gpu::Device dev(p.N, p.precision, p.device);
+    Obs chainObs(&dev, false, p.NCf, 8u), flowObs(&dev, true, p.NFlows + 1, 8u);
+    Correlator *corr = &chainObs;  // GLAC's m_correlator is a Correlator*
+    Obs *flowCorr = &flowObs;
+    {
+        glac_gpu::System<Lattice<SU3>, Correlator, Obs> system(&dev, p, corr, flowCorr);
+        glac_gpu::System<Lattice<SU3>, Correlator, Obs> *sys = &system;
+        sys->latticeSetup();
+        sys->run();
+
+        // the oracle's chain
+        std::vector<double> U(4 * V * 18), W, qt(8);
+        or_hot_start(U.data(), n, 41, OR_HOT_GLAC_RANDOM, 0.24);
+        unsigned long long sweep = 0;
+        for (; sweep < 3; sweep++) or_cb_sweep(U.data(), n, 6.0, 0.24, 10, 31, sweep, 1);
+        double acc = 0.0, out[3];
+        for (unsigned c = 0; c < p.NCf; c++) {
+            for (unsigned k = 0; k < p.NCor; k++) acc += or_cb_sweep(U.data(), n, 6.0, 0.24, 10, 31, sweep++, 1);
+            W = U;
+            REQUIRE(flowCorr->m_flowFiles.size() > c, "no flow file for configuration %u", c);
+            for (unsigned i = 0; i <= p.NFlows; i++) {
+                if (i > 0) or_flow_step(W.data(), n, 0.02);
+                or_measure(W.data(), n, out, qt.data());
+                const std::vector<double> &o = flowCorr->m_flowFiles[c][i];
+                REQUIRE(std::fabs(o[0] - out[0]) < 1e-12, "config %u flow %u plaq %g vs %g", c, i, o[0], out[0]);
+                REQUIRE(std::fabs(o[1] - out[2]) < 1e-12, "config %u flow %u topc", c, i);
+                REQUIRE(std::fabs(o[2] / out[1] - 1.0) < 1e-11, "config %u flow %u energy", c, i);
+                for (int t = 0; t < 8; t++) REQUIRE(std::fabs(o[3 + t] - qt[t]) < 1e-12, "config %u topct", c);
+            }
+            const std::vector<double> o0 = corr->getObservablesVector(c);
+            REQUIRE(o0 == flowCorr->m_flowFiles[c][0], "chain observable %u is not the flow's t = 0 row", c);
+        }
+        acc /= double(p.NCf * p.NCor);
+        REQUIRE(std::fabs(sys->getAcceptanceRate() - acc) < 1e-12, "acceptance %.15g vs %.15g",
+                sys->getAcceptanceRate(), acc);
+        REQUIRE(std::fabs(corr->m_acceptanceWritten - acc) < 1e-12, "acceptance handed to writeObservableToFile");
+        // the chain after the run, det = -1 links included, and its last configuration file
+        sys->syncHost();
+        double e = 0, d[2], dmax = 0;
+        for (int mu = 0; mu < 4; mu++)
+            for (unsigned long s = 0; s < V; s++) {
+                for (int k = 0; k < 18; k++)
+                    e = std::fmax(e, std::fabs(sys->lattice()[mu].m_sites[s].mat[k] - U[(mu * V + s) * 18 + k]));
+                or_su3_det(sys->lattice()[mu].m_sites[s].mat, d);
+                dmax = std::fmax(dmax, std::fabs(d[0] + 1.0) + std::fabs(d[1]));
+            }
+        REQUIRE(e < 1e-10, "chain after the run differs from the oracle by %g", e);
+        REQUIRE(dmax < 1e-12, "det = -1 not kept: %g", dmax);
+        FILE *fp = std::fopen((dir + "/system_config00002.bin").c_str(), "rb");
+        REQUIRE(fp, "configuration file not written");
+        std::vector<double> file(V * 72);
+        const size_t got = std::fread(file.data(), sizeof(double), file.size(), fp);
+        std::fclose(fp);
+        REQUIRE(got == file.size(), "short configuration file");
+        e = 0;
+        for (unsigned long s = 0; s < V; s++)
+            for (int mu = 0; mu < 4; mu++)
+                for (int k = 0; k < 18; k++)
+                    e = std::fmax(e, std::fabs(file[(s * 4 + mu) * 18 + k] - U[(mu * V + s) * 18 + k]));
+        REQUIRE(e < 1e-10, "configuration file differs from the oracle chain by %g", e);
+    }
+    std::printf("PASS System run loop (GLAC random start, %u + %u x %u sweeps, %u flow steps per configuration)\n",
+                p.NTherm, p.NCf, p.NCor, p.NFlows);
     return 0;
 }
 
@@ -224,6 +378,10 @@ int main(int argc, char **argv)
     if (argc < 2) return 2;
     const std::string mode = argv[1];
     if (mode == "cpu") return run_cpu();
-    if (mode == "gpu") return run_gpu(argc > 2 ? argv[2] : "/tmp");
+    if (mode == "gpu") {
+        const std::string dir = argc > 2 ? argv[2] : "/tmp";
+        const int rc = run_gpu(dir);
+        return rc ? rc : run_system(dir);
+    }
     return 2;
 }

@@ -56,3 +56,32 @@ def test_det_minus_one_flow_and_measurement_follow_the_oracle(prec):
     assert np.abs(q - wq).max() < tol_q
     assert np.abs(got - W.reshape(4, -1, 18)).max() < tol_u
     assert np.abs(det + 1).max() < 10 * tol_u
+
+
+@pytest.mark.parametrize("prec", [glacx.FP64, glacx.FP32_LINKS], ids=["fp64", "fp32"])
+def test_det_minus_one_sweeps_with_reunitarisation_keep_det(prec):
+    """Metropolis sweeps with reunitarisation on a det = -1 field keep
+    det U = -1, and (FP64) make the oracle's accept/reject decisions on the
+    same streams."""
+    n = (4, 4, 4, 8)
+    print(f"\n[gpu case] det = -1 sweeps with reunitarisation {n} prec {prec}", flush=True)
+    sys.stdout.flush()
+    U = glac_started_field(n)
+    W = U.copy()
+    accs = [O.cb_sweep(W, n, 6.0, 0.24, 10, 31, s, True) for s in range(3)]
+    with glacx.Lattice(n, prec) as L:
+        L.upload(U)
+        got_acc = L.metropolis(3, 6.0, 0.24, 10, reunit_every=1, seed=31)
+        got = L.download()
+    det = np.linalg.det(O.to_complex(got).reshape(-1, 3, 3))
+    want_det = np.linalg.det(O.to_complex(W).reshape(-1, 3, 3))
+    assert np.abs(want_det + 1).max() < 1e-13
+    if prec == glacx.FP64:
+        assert np.abs(det + 1).max() < 1e-13
+        nl = 4 * 10 * np.prod(n)
+        assert np.array_equal(np.round(np.array(accs) * nl), np.round(got_acc * nl))
+        assert np.abs(got - W).max() < 1e-10
+    else:
+        assert np.abs(det + 1).max() < 1e-5
+        close = np.abs(got - W).reshape(4, -1, 18).max(axis=-1) < 1e-4
+        assert close.mean() > 0.98

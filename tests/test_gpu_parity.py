@@ -59,6 +59,28 @@ def test_hot_start_matches_oracle(prec):
     assert np.abs(got - want).max() < tol
 
 
+@pytest.mark.parametrize("kind", [glacx.HOT_GLAC_RANDOM, glacx.HOT_RST], ids=["glac_random", "rst"])
+def test_hot_start_kinds_match_oracle(prec, kind):
+    """GLAC's two hot starts (generateRandom, det = -1; RSTHotStart) drawn
+    from the link streams, against the oracle on the same streams."""
+    n = (4, 6, 4, 8)
+    banner(f"hot start kind {kind} {n} {PNAME[prec]}")
+    with glacx.Lattice(n, prec) as L:
+        if kind == glacx.HOT_GLAC_RANDOM and prec == glacx.FP32_COMPRESSED:
+            with pytest.raises(glacx.GlacxError, match="det U = -1"):
+                L.hot_start(77, kind=kind, eps=0.24)
+            return
+        L.hot_start(77, kind=kind, eps=0.24)
+        got = L.download()
+        p, _, _, _ = L.measure()
+    want = O.hot_start(n, 77, kind, 0.24).reshape(4, -1, 18)
+    tol = {glacx.FP64: 1e-13, glacx.FP32_LINKS: 1e-7, glacx.FP32_COMPRESSED: 5e-7}[prec]
+    assert np.abs(got - want).max() < tol
+    det = np.linalg.det(O.to_complex(got).reshape(-1, 3, 3))
+    assert np.abs(det - (-1.0 if kind == glacx.HOT_GLAC_RANDOM else 1.0)).max() < 10 * tol + 1e-13
+    assert abs(p - O.measure(want, n)[0]) < (1e-13 if prec == glacx.FP64 else 5e-7)
+
+
 def test_cold_start_measures_unity(prec):
     n = (4, 4, 4, 4)
     banner(f"cold start {n} {PNAME[prec]}")
@@ -147,33 +169,52 @@ def test_flow_matches_oracle_per_step(n, steps, spread, prec):
         assert np.abs(q - wq).max() < 1e-12
         assert np.abs(qt - wqt).max() < 1e-12
     else:
+        de = np.abs(e / we - 1).max()
+        print(f"fp32 flow {n}: max |dE|/E {de:.2e}, |dQ| {np.abs(q - wq).max():.2e}, "
+              f"|dU| {np.abs(got - W).max():.2e}")
         assert np.abs(got - W).max() < 5e-6
         assert np.abs(p - wp).max() < 5e-7
-        assert np.abs(e / we - 1).max() < 1e-5
+        assert de < 1e-6
         assert np.abs(q - wq).max() < 1e-4
 
 
-def test_flow_8x8x8x16_against_oracle(prec):
-    """8^3 x 16: a rough Metropolis-like field flowed 100 RK3 steps at
-    eps = 0.01; E(t) and Q(t) per step."""
+@pytest.fixture(scope="module")
+def config_8x8x8x16():
+    """8^3 x 16, beta = 6.0: Haar hot start and 200 FP64 device sweeps
+    (eps = 0.24, 10 hits), then 100 RK3 steps of the oracle at eps = 0.01
+    with the clover observables per step."""
     n = (8, 8, 8, 16)
-    banner(f"flow {n} 100 steps {PNAME[prec]}")
-    U = O.hot_start_haar(n, 99)
-    for s in range(4):
-        O.cb_sweep(U, n, 6.0, 0.24, 10, 99, s)
+    banner(f"config {n}: 200 fp64 device sweeps + oracle flow")
+    with glacx.Lattice(n, glacx.FP64) as L:
+        L.hot_start(99)
+        acc, plaq = L.metropolis(200, 6.0, 0.24, 10, reunit_every=1, seed=99, plaquette=True)
+        U = np.ascontiguousarray(L.download())
     W, rows = run_oracle_flow(U, n, 0.01, 100)
+    return n, U, acc, plaq, rows
+
+
+def test_flow_8x8x8x16_against_oracle(prec, config_8x8x8x16):
+    """8^3 x 16 after 200 device sweeps, flowed 100 RK3 steps at eps = 0.01:
+    plaquette, E(t) and Q(t) per step against the oracle."""
+    n, U, acc, plaq, rows = config_8x8x8x16
+    banner(f"flow {n} 100 steps {PNAME[prec]}")
+    assert 0.55 < plaq[-50:].mean() < 0.62 and 0.05 < acc.mean() < 0.2
     with glacx.Lattice(n, prec) as L:
         L.upload(U)
         L.copy(glacx.FLOW, glacx.MAIN)
         p, e, q, qt = L.flow(100, 0.01)
+    wp = np.array([r[0] for r in rows])
     we = np.array([r[1] for r in rows])
     wq = np.array([r[2] for r in rows])
+    wqt = np.array([r[3] for r in rows])
+    de, dq = np.abs(e / we - 1).max(), np.abs(q - wq).max()
+    print(f"{PNAME[prec]} flow {n}: max |dE|/E {de:.2e}, |dQ| {dq:.2e}, |dplaq| {np.abs(p - wp).max():.2e}")
     if prec == glacx.FP64:
-        assert np.abs(e / we - 1).max() < 1e-11
-        assert np.abs(q - wq).max() < 1e-11
+        assert de < 1e-11 and dq < 1e-11
+        assert np.abs(p - wp).max() < 1e-12 and np.abs(qt - wqt).max() < 1e-11
     else:
-        assert np.abs(e / we - 1).max() < 1e-5
-        assert np.abs(q - wq).max() < 1e-4
+        assert de < 1e-6 and dq < 1e-4
+        assert np.abs(p - wp).max() < 5e-7
 
 
 def test_flow_keeps_pure_gauge(prec):

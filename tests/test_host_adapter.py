@@ -30,14 +30,15 @@ def test_delta_action_through_adapter():
     exe = build_check()
     out = subprocess.run([exe, "cpu"], capture_output=True, text=True)
     assert out.returncode == 0, out.stdout + out.stderr
-    assert "PASS dS = -36" in out.stdout
+    assert "PASS dS = -36 on the unit field" in out.stdout
+    assert "PASS dS = -36 through GLAC's Action interface" in out.stdout
 
 
 @pytest.mark.gpu
 def test_flow_measure_metropolis_through_adapter(tmp_path):
     exe = build_check()
-    print("\n[gpu case] host adapter flow/measure/metropolis 4x4x4x8 fp64", flush=True)
+    print("\n[gpu case] host adapter flow/measure/metropolis/System loop 4x4x4x8 fp64", flush=True)
     out = subprocess.run([exe, "gpu", str(tmp_path)], capture_output=True, text=True, timeout=120)
     print(out.stdout)
     assert out.returncode == 0, out.stdout + out.stderr
-    assert out.stdout.count("PASS") == 5
+    assert out.stdout.count("PASS") == 6 and "PASS System run loop" in out.stdout

@@ -125,3 +125,20 @@ def test_reunitarize_gives_su3_and_is_idempotent():
         assert np.allclose(R[0], m[0] / np.linalg.norm(m[0]), atol=1e-14)
         R2 = O.to_complex(O.reunitarize(O.to_pairs(R)))
         assert np.allclose(R2, R, atol=1e-14)
+
+
+def test_reunitarize_keep_det_sign():
+    """det = -1 links (GLAC's generateRandom start) stay det = -1 and close
+    to the input; det = +1 links get or_reunitarize's result exactly."""
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        H = O.to_complex(O.generate_random(rng.uniform(-0.24, 0.24, size=18)))
+        noisy = H + 1e-6 * (rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3)))
+        R = O.to_complex(O.reunitarize_keep_det_sign(O.to_pairs(noisy)))
+        assert np.allclose(R @ R.conj().T, np.eye(3), atol=1e-14)
+        assert abs(np.linalg.det(R) + 1.0) < 1e-14
+        assert np.abs(R - H).max() < 1e-5
+        m = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+        S = O.to_complex(O.reunitarize(O.to_pairs(m)))
+        S_noisy = O.to_pairs(S + 1e-6 * rng.normal(size=(3, 3)))
+        assert np.array_equal(O.reunitarize_keep_det_sign(S_noisy), O.reunitarize(S_noisy))

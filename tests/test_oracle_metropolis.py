@@ -110,3 +110,31 @@ def test_glac_random_hot_start_is_unitary_with_det_minus_one():
     M = O.to_complex(U).reshape(-1, 3, 3)
     assert np.abs(M @ np.conj(np.swapaxes(M, -1, -2)) - np.eye(3)).max() < 1e-14
     assert np.abs(np.linalg.det(M) + 1).max() < 1e-14
+
+
+def _moments(U):
+    M = O.to_complex(U).reshape(-1, 3, 3)
+    tr = np.trace(M, axis1=-2, axis2=-1)
+    return M, np.array([tr.real.mean(), (np.abs(tr) ** 2).mean(), (M[:, 0, 0].real ** 2).mean()]), len(tr)
+
+
+@pytest.mark.parametrize("rst", [False, True], ids=["glac_random", "rst"])
+def test_stream_hot_starts_sample_glacs_distributions(rst):
+    """The device's GLAC-kind hot starts (link streams) against GLAC's own
+    generators on its mt19937_64 stream (or_glac_hot_start): same law, so the
+    same moments within statistical error; unitarity and det = -1 / +1."""
+    n = (8, 8, 8, 8)
+    kind = O.HOT_RST if rst else O.HOT_GLAC_RANDOM
+    A = O.hot_start(n, 2025, kind, 0.24)
+    B = O.new_field(n)
+    O.glac_hot_start(B, n, O.GlacRng(5, 6), 0.24, rst=rst)
+    Ma, ma, N = _moments(A)
+    Mb, mb, _ = _moments(B)
+    assert np.abs(Ma @ np.conj(np.swapaxes(Ma, -1, -2)) - np.eye(3)).max() < 1e-14
+    assert np.abs(np.linalg.det(Ma) - (1.0 if rst else -1.0)).max() < 1e-14
+    assert np.all(np.abs(ma - mb) < 6 / np.sqrt(N)), (ma, mb)
+
+
+def test_stream_haar_kind_is_the_haar_start():
+    n = (4, 4, 4, 6)
+    assert np.array_equal(O.hot_start(n, 9, O.HOT_HAAR), O.hot_start_haar(n, 9))
