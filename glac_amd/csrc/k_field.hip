@@ -142,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void action_derivative_kernel(const typenam
 {
     const int idx = blockIdx.x * kBlock + threadIdx.x;
     if (idx >= g.v) return;
-    const Coord x = decode(g, idx);
+    const Site x = site_of(g, idx);
     const M3<T> a = staple<T, R>(f, g, x, mu);
     const M3<T> u = load_link<T, R>(f, g, mu, idx);
     const M3<T> z = expand(force_from_omega(mul(u, a)));

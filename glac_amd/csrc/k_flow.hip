@@ -16,7 +16,8 @@
 // The accumulator holds the 8 real parameters of an su(3) element per link
 // (16-byte vectors, see acc_store).
 //
-// FP32 links are reunitarised (FP64 Gram-Schmidt) as they are stored.
+// FP32 links are reunitarised (FP64 Gram-Schmidt) as the last stage of a
+// step stores them.
 //
 // Launch: 256-thread blocks = 64 sites x 4 directions; wavefront w handles
 // mu = w for the block's 64 sites, the h-th to (h+31)-th of each parity (two
@@ -200,13 +201,13 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : 2) void f
     const bool valid = h_raw < g.vh;
     const int h = valid ? h_raw : g.vh - 1;  // idle lanes of a partial block mirror a real site, store nothing
     const int idx = parity_index(g, lane >> 5, h);
-    const Coord x = decode(g, idx);
+    const Site x = site_of(g, idx);
     const DirView<T> vmu = dir_view<T>(win, g, mu);
     // the site's neighbours x + mu and x - mu, shared with the other
     // wavefronts, whose staples step the same way in direction mu
     __shared__ int nbr[2][4][kFlowSites];
-    const Coord xpm = step(g, x, mu, +1);
-    const int ipm = index_of(g, xpm), imm = index_of(g, step(g, x, mu, -1));
+    const Site xpm = site_step(g, x, mu, +1);
+    const int ipm = xpm.idx, imm = site_step(g, x, mu, -1).idx;
     nbr[0][mu][lane] = ipm;
     nbr[1][mu][lane] = imm;
     lds_put(sh, mu, lane, load_link<T, R>(vmu, idx));
@@ -224,7 +225,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : 2) void f
             const int nu = MEAS ? (int)((kPhaseNu >> (4 * e)) & 0xF) : (mu + k + 1) & 3;
             const DirView<T> vnu = dir_view<T>(win, g, nu);
             const int ipn = nbr[0][nu][lane], imn = nbr[1][nu][lane];
-            const int ipmmn = index_of(g, step(g, xpm, nu, -1));
+            const int ipmmn = site_step(g, xpm, nu, -1).idx;
             if constexpr (MEAS) {
                 // the same staple terms, kept apart for the clover leaves
                 M3<T> up, low;
@@ -260,7 +261,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : 2) void f
                 }
                 if ((kPhaseL3 >> e) & 1) {
                     // L3 = (U_d2(y) U_d1(x-d1))^dag (U_d1(y) U_d2(x-d2)),  y = x - d1 - d2
-                    const int iy = index_of(g, step(g, step(g, x, d1, -1), d2, -1));
+                    const int iy = site_step(g, site_step(g, x, d1, -1), d2, -1).idx;
                     const DirView<T> v1 = first ? vmu : vnu, v2 = first ? vnu : vmu;
                     M3<T> m1, m2;
                     if constexpr (R == 2) {
@@ -334,14 +335,17 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : 2) void f
         }
         M3<T> w = zero3<T>();
         mm<false, false, false, R>(w, exp_su3(y), u);  // two rows suffice when two are stored
-        if constexpr (sizeof(T) == 4) {
-            // FP32 links: project back onto the group in FP64 every stage.
-            // Without it the rounding of exp(Y) W drifts the links off it
-            // (|U U^dag - 1| ~ 1e-4 after 1000 steps at 24^3 x 48), and that drift,
-            // not the FP32 measurement, dominates the error of E(t) at large t.
-            // Three stored rows are orthonormalised row by row, which keeps
-            // det U (GLAC's det = -1 starts stay det = -1); two stored rows are
-            // SU(3) and the third is rebuilt on load.
+        if constexpr (sizeof(T) == 4 && STAGE == 2) {
+            // FP32 links: project back onto the group in FP64 once per flow
+            // step, as the last stage stores them.  Without it the rounding of
+            // exp(Y) W drifts the links off it (|U U^dag - 1| ~ 1e-4 after 1000
+            // steps at 24^3 x 48), and that drift, not the FP32 measurement,
+            // dominates the error of E(t) at large t; within a step the two
+            // unprojected stores add two roundings (~1e-7), which the
+            // projection then removes.  Three stored rows are orthonormalised
+            // row by row, which keeps det U (GLAC's det = -1 starts stay
+            // det = -1); two stored rows are SU(3) and the third is rebuilt on
+            // load.
             M3<double> d = convert<double>(w);
             if constexpr (R == 2)
                 reunitarize<2, true>(d);

@@ -46,7 +46,7 @@ __global__ __launch_bounds__(kBlock) void collect_kernel(const typename vec2<T>:
     const size_t s = base + (size_t)(h / stride);
     if (s >= cap) return;
     const int idx = parity_index(g, parity, h);
-    const Coord x = decode(g, idx);
+    const Site x = site_of(g, idx);
     const M3<T> a = staple<T, R>(f, g, x, mu);
     const M3<T> u = load_link<T, R>(f, g, mu, idx);
 #pragma unroll
@@ -328,7 +328,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 3 : 1) void metropolis_lea
     double nacc[1] = {0.0};
     {
         const int idx = parity_index(g, parity, h);
-        const Coord x = decode(g, idx);
+        const Site x = site_of(g, idx);
         const M3<T> a = staple<T, R>(f, g, x, mu);
         M3<T> u = load_link<T, R>(f, g, mu, idx);
         float fe[6], o[36], L[36];

@@ -34,7 +34,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 8 ? 2 : 1) void metropolis_ker
     double nacc[1] = {0.0};
     if (h < g.vh) {
         const int idx = parity_index(g, parity, h);
-        const Coord x = decode(g, idx);
+        const Site x = site_of(g, idx);
         const M3<T> a = staple<T, R>(f, g, x, mu);
         M3<T> u = load_link<T, R>(f, g, mu, idx);
         Stream rs(prm.seed, (uint32_t)lex_of(g, x), (uint32_t)mu, STREAM_METROPOLIS, prm.sweep);

@@ -75,6 +75,10 @@ struct Geom {
     int vs;         // sites per time slice, Nx Ny Nz
     int cpr;        // 32-site chunks per run, ceil(run / 32)
     FastDiv d_nxh, d_ny, d_nzb, d_run, d_bh, d_2bh, d_cpr;
+    // storage offsets of a step in y, z, t (index mu - 1), backwards [0] or
+    // forwards [1], apart from the parity flip: within the block [0], into
+    // the next block [1], across the periodic boundary [2] (site_step)
+    int so[3][2][3];
 };
 
 // log2 of the preferred z and t block extents (a build-time tuning knob);
@@ -107,6 +111,21 @@ __host__ inline Geom make_geom(const int dims[4])
     g.d_run = make_fastdiv(g.run);
     g.d_bh = make_fastdiv(g.bh);
     g.d_2bh = make_fastdiv(2 * g.bh);
+    const int bz = 1 << g.lbz, bt = 1 << g.lbt, plane = dims[1] * g.nxh, ntb = dims[3] >> g.lbt;
+    for (int f = 0; f < 2; f++) {
+        const int s = f ? 1 : -1;
+        // y: one row; across the boundary the other end of the plane
+        g.so[0][f][0] = s * g.nxh;
+        g.so[0][f][1] = g.so[0][f][2] = -s * (dims[1] - 1) * g.nxh;
+        // z: one plane; into the next z-block (2 BH further) at its other end
+        g.so[1][f][0] = s * plane;
+        g.so[1][f][1] = -s * (bz - 1) * plane + s * 2 * g.bh;
+        g.so[1][f][2] = -s * (bz - 1) * plane - s * 2 * g.bh * (g.nzb - 1);
+        // t: one run of bz planes; into the next t-block (nzb z-blocks further)
+        g.so[2][f][0] = s * g.run;
+        g.so[2][f][1] = -s * (bt - 1) * g.run + s * 2 * g.bh * g.nzb;
+        g.so[2][f][2] = -s * (bt - 1) * g.run - s * 2 * g.bh * g.nzb * (ntb - 1);
+    }
     return g;
 }
 
@@ -187,6 +206,84 @@ __host__ __device__ inline int slice_index(const Geom &g, int t, int r)
     const int j = r - q * g.run;
     const int blk = (q >> 1) + g.nzb * (t >> g.lbt);
     return (blk * 2 + (q & 1)) * g.bh + (t & ((1 << g.lbt) - 1)) * g.run + j;
+}
+
+// ---- neighbour steps without multiplies ------------------------------------
+// A site as its storage index plus the coordinates a step needs in the slab
+// layout: x, y, the z and t offsets inside the block (lz, lt), the block's z
+// and t indices (zb, tb) and the parity p.  A step flips the parity (the
+// other half of the same block, +-BH) and moves within the block by a
+// wave-uniform stride -- 1 per x pair, Nx/2 per y row, Ny Nx/2 per z plane,
+// bz Ny Nx/2 per t slice -- or into the neighbouring block; every offset is a
+// wave-uniform constant chosen per lane by a compare, so a step costs a few
+// adds and selects where index_of(step(...)) costs three 32-bit multiplies
+// (quarter-rate instructions).
+struct Site {
+    int idx, x, y, lz, zb, lt, tb, p;
+};
+
+__host__ __device__ inline Site site_of(const Geom &g, int idx)
+{
+    const int blk = fdiv(idx, g.d_2bh);
+    int j = idx - blk * 2 * g.bh;
+    const int p = j >= g.bh;
+    j -= p * g.bh;
+    const int q1 = fdiv(j, g.d_nxh);
+    const int hx = j - q1 * g.nxh;
+    const int q2 = fdiv(q1, g.d_ny);
+    Site s;
+    s.idx = idx;
+    s.y = q1 - q2 * g.n[1];
+    s.lz = q2 & ((1 << g.lbz) - 1);
+    s.lt = q2 >> g.lbz;
+    s.tb = fdiv(blk, g.d_nzb);
+    s.zb = blk - s.tb * g.nzb;
+    s.p = p;
+    const int z = (s.zb << g.lbz) + s.lz, t = (s.tb << g.lbt) + s.lt;
+    s.x = 2 * hx + ((s.y + z + t + p) & 1);
+    return s;
+}
+
+// GLAC's lexicographic index x + Nx (y + Ny (z + Nz t)) of a site
+__host__ __device__ inline int lex_of(const Geom &g, const Site &a)
+{
+    const int z = (a.zb << g.lbz) + a.lz, t = (a.tb << g.lbt) + a.lt;
+    return a.x + g.n[0] * (a.y + g.n[1] * (z + g.n[2] * t));
+}
+
+// the site one step from `a` in direction mu (s = +1 or -1), periodic
+__host__ __device__ inline Site site_step(const Geom &g, const Site &a, int mu, int s)
+{
+    Site b = a;
+    b.p = a.p ^ 1;
+    int off = a.p ? -g.bh : g.bh;
+    const int f = s > 0;
+    if (mu == 0) {
+        int x = a.x + s;
+        x = x < 0 ? g.n[0] - 1 : (x == g.n[0] ? 0 : x);
+        b.x = x;
+        off += (x >> 1) - (a.x >> 1);
+    } else if (mu == 1) {
+        const bool wrap = s > 0 ? a.y == g.n[1] - 1 : a.y == 0;
+        b.y = wrap ? (s > 0 ? 0 : g.n[1] - 1) : a.y + s;
+        off += wrap ? g.so[0][f][2] : g.so[0][f][0];
+    } else if (mu == 2) {
+        const int bz = 1 << g.lbz;
+        const bool cross = s > 0 ? a.lz == bz - 1 : a.lz == 0;
+        const bool wrap = s > 0 ? a.zb == g.nzb - 1 : a.zb == 0;
+        b.lz = cross ? (s > 0 ? 0 : bz - 1) : a.lz + s;
+        b.zb = cross ? (wrap ? (s > 0 ? 0 : g.nzb - 1) : a.zb + s) : a.zb;
+        off += cross ? (wrap ? g.so[1][f][2] : g.so[1][f][1]) : g.so[1][f][0];
+    } else {
+        const int bt = 1 << g.lbt, ntb = g.n[3] >> g.lbt;
+        const bool cross = s > 0 ? a.lt == bt - 1 : a.lt == 0;
+        const bool wrap = s > 0 ? a.tb == ntb - 1 : a.tb == 0;
+        b.lt = cross ? (s > 0 ? 0 : bt - 1) : a.lt + s;
+        b.tb = cross ? (wrap ? (s > 0 ? 0 : ntb - 1) : a.tb + s) : a.tb;
+        off += cross ? (wrap ? g.so[2][f][2] : g.so[2][f][1]) : g.so[2][f][0];
+    }
+    b.idx = a.idx + off;
+    return b;
 }
 
 // coordinates of r + s * e_mu (s = +-1), periodic.  Written with selects over
@@ -404,20 +501,20 @@ __device__ __forceinline__ void store_link_any(typename vec2<T>::type *__restric
 //                                        + U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu) ]
 // (GLAC/actions/wilsongaugeaction.cpp:29-53).
 template <class T, int R>
-__device__ __forceinline__ M3<T> staple(const typename vec2<T>::type *__restrict__ f, const Geom &g, const Coord &x,
+__device__ __forceinline__ M3<T> staple(const typename vec2<T>::type *__restrict__ f, const Geom &g, const Site &x,
                                         int mu)
 {
     M3<T> a = zero3<T>();
-    const Coord xpm = step(g, x, mu, +1);
-    const int ix = index_of(g, x), ipm = index_of(g, xpm);
+    const Site xpm = site_step(g, x, mu, +1);
+    const int ix = x.idx, ipm = xpm.idx;
     const DirView<T> vmu = dir_view<T>(f, g, mu);
 #pragma unroll 1
     for (int d = 1; d < 4; d++) {
         const int nu = (mu + d) & 3;
         const DirView<T> vnu = dir_view<T>(f, g, nu);
-        const int ipn = index_of(g, step(g, x, nu, +1));
-        const int imn = index_of(g, step(g, x, nu, -1));
-        const int ipmmn = index_of(g, step(g, xpm, nu, -1));
+        const int ipn = site_step(g, x, nu, +1).idx;
+        const int imn = site_step(g, x, nu, -1).idx;
+        const int ipmmn = site_step(g, xpm, nu, -1).idx;
         if constexpr (R == 2) {
             // SU(3) links (the compressed mode checks them): each term's
             // partial products keep two rows, the term's third is rebuilt

@@ -34,6 +34,25 @@ static void check(const int dims[4])
         const int lex = lex_of(g, c);
         REQUIRE(lex >= 0 && lex < V && !seen[lex], "lex %d of idx %d", lex, idx);
         seen[lex] = 1;
+        // the multiply-free neighbour steps agree with index_of(step(...)),
+        // also for two steps in a row (the staple's x + mu - nu)
+        const Site st = site_of(g, idx);
+        REQUIRE(st.idx == idx && st.p == ((c.c[0] + c.c[1] + c.c[2] + c.c[3]) & 1), "site_of(%d)", idx);
+        for (int mu = 0; mu < 4; mu++)
+            for (int s = -1; s <= 1; s += 2) {
+                const Site a = site_step(g, st, mu, s);
+                const Coord cn = step(g, c, mu, s);
+                REQUIRE(a.idx == index_of(g, cn), "site_step(%d, mu %d, s %d) = %d, want %d", idx, mu, s, a.idx,
+                        index_of(g, cn));
+                const Site ac = site_of(g, a.idx);
+                REQUIRE(ac.x == a.x && ac.y == a.y && ac.lz == a.lz && ac.zb == a.zb && ac.lt == a.lt &&
+                            ac.tb == a.tb && ac.p == a.p,
+                        "site_step(%d, mu %d, s %d) coordinates", idx, mu, s);
+                for (int nu = 0; nu < 4; nu++)
+                    for (int r = -1; r <= 1; r += 2)
+                        REQUIRE(site_step(g, a, nu, r).idx == index_of(g, step(g, cn, nu, r)),
+                                "two steps from %d: mu %d s %d, nu %d r %d", idx, mu, s, nu, r);
+            }
         for (int mu = 0; mu < 4; mu++)
             for (int s = -1; s <= 1; s += 2) {
                 const Coord n = step(g, c, mu, s);
