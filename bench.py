@@ -14,8 +14,10 @@ Also reported, each from its own timed region:
    sweep) and FP64 flow steps/s;
  - ``roofline``: the flow-stage kernel's algorithmic bytes per launch over its
    mean launch time (HIP events on the library's stream), against 8 TB/s, with
-   the HBM traffic per launch measured by rocprofv3 (profiles/).  With the
-   measurement fused into stage 0 (FP32 links), that is RK3 stages 1 and 2;
+   the HBM traffic per launch measured by rocprofv3 and the fraction of the
+   launch the SIMDs spend issuing its VALU instructions (rocprofv3 SQ
+   counters per wavefront, profiles/counters_<lattice>.json).  With the
+   measurement fused into stage 0, that is RK3 stages 1 and 2;
    ``stage0_measure`` reports the fused stage 0 the same way;
  - ``indep_configs_per_s``: effectively independent configurations per second,
    configs/s / (2 tau_int(Q^2)), for the learned proposal with FP32 links,
@@ -67,16 +69,32 @@ def metropolis_bytes_per_launch(V, fp32, rows=3):
     return 4.5 * V * link_bytes(fp32, rows)
 
 
-def profiled_traffic(n, kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE measurement (profiles/traffic_<lattice>.json),
-    or None when that lattice was not profiled."""
-    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % "x".join(map(str, n)))
+SIMDS, CLOCK_HZ, VALU_CYCLES = 1024, 2.4e9, 4  # MI355X: 256 CUs x 4 SIMDs; a wave64 VALU op holds a SIMD 4 cycles
+
+
+def profiled(n, kernels):
+    """Mean over `kernels` of the committed rocprofv3 figures per launch
+    (profiles/counters_<lattice>.json: SQ counters per wavefront, HBM bytes
+    from FETCH_SIZE / WRITE_SIZE), or None when the lattice or a kernel was
+    not profiled."""
+    path = os.path.join(ROOT, "profiles", "counters_%s.json" % "x".join(map(str, n)))
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        entry = json.load(f).get(kernel)
-    return entry["traffic_bytes_per_launch"] if entry else None
+        table = json.load(f)["kernels"]
+    rows = [table.get(k) for k in kernels]
+    if not all(rows):
+        return None
+    return {key: sum(r[key] for r in rows) / len(rows) for key in ("waves", "valu_per_wave", "traffic_bytes")}
+
+
+def valu_issue_frac(prof, launch_ms):
+    """Fraction of the launch time the SIMDs spend issuing the kernel's VALU
+    instructions: waves x VALU instructions per wave x 4 cycles over
+    1024 SIMDs x 2.4 GHz x the launch time measured in this run."""
+    if prof is None:
+        return None
+    return prof["waves"] * prof["valu_per_wave"] * VALU_CYCLES / (SIMDS * CLOCK_HZ * launch_ms * 1e-3)
 
 
 VARIANTS = {"fp32c_learned": ("FP32_COMPRESSED", "learned"), "fp32c_rst": ("FP32_COMPRESSED", "rst"),
@@ -265,6 +283,9 @@ def main():
     metro_bytes = metropolis_bytes_per_launch(V, True, rows)
     metro_ms = m_ms / max(m_launches, 1)
     metro_gbs = metro_bytes / (metro_ms * 1e-3) / 1e9
+    prof_stage = profiled(n, [f"flow_stage_kernel<float, {rows}, {k}, false>" for k in plain_stages])
+    prof_stage0 = profiled(n, [f"flow_stage_kernel<float, {rows}, 0, true>"])
+    prof_metro = profiled(n, [f"metropolis_kernel<float, {rows}>"])
 
     # ---------------------------------------------------------------- FP64 flow
     fp64 = None
@@ -333,18 +354,20 @@ def main():
                    "beta": beta, "flow_eps": eps_flow, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(n, f"flow_stage_kernel<float, {rows}>"),
+                     "traffic": prof_stage["traffic_bytes"] if prof_stage else None,
                      "kernel": f"flow_stage_kernel<float, {rows}, {{{','.join(map(str, plain_stages))}}}, false>",
-                     "alg_bytes_per_launch": stage_bytes, "mean_launch_ms": stage_ms},
+                     "alg_bytes_per_launch": stage_bytes, "mean_launch_ms": stage_ms,
+                     "valu_issue_frac": valu_issue_frac(prof_stage, stage_ms)},
         "metropolis": {"sweeps_per_s": sweeps_per_s, "acceptance": float(acc.mean()),
                        "subsweep_kernel_ms": metro_ms, "subsweep_kernel_gbs": metro_gbs,
-                       "hbm_frac": metro_gbs / HBM_PEAK_GBS},
+                       "hbm_frac": metro_gbs / HBM_PEAK_GBS, "valu_issue_frac": valu_issue_frac(prof_metro, metro_ms)},
         "measure_kernel_ms": meas_ms / max(meas_launches, 1),
         "flow_steps_per_s_with_kernel_events": args.steps / elapsed_events,
         "stage0_measure": ({"kernel": f"flow_stage_kernel<float, {rows}, 0, true>", "mean_launch_ms": fm_ms / fm_launches,
                             "alg_bytes_per_launch": stage0_bytes,
                             "achieved_gbs": stage0_bytes / (fm_ms / fm_launches * 1e-3) / 1e9,
-                            "traffic": profiled_traffic(n, f"flow_stage_kernel<float, {rows}, 0, true>")}
+                            "traffic": prof_stage0["traffic_bytes"] if prof_stage0 else None,
+                            "valu_issue_frac": valu_issue_frac(prof_stage0, fm_ms / fm_launches)}
                            if fm_launches else None),
         "fp64_plain": fp64,
         "indep_configs_per_s": indep,

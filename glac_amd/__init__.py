@@ -83,6 +83,7 @@ def library():
         "glacx_proposal_get": ([_vp, _fp, _fp, _fp], C.c_int),
         "glacx_train": ([_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint64, _dp], C.c_int),
         "glacx_train_grad": ([_vp, _fp, C.c_int, _fp, C.c_int, C.c_double, _dp, _fp], C.c_int),
+        "glacx_proposal_eval": ([_vp, _fp, C.c_int, _fp, C.POINTER(C.c_int)], C.c_int),
         "glacx_metropolis_learned": ([_vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_uint64,
                                       C.c_uint64, _dp, _dp], C.c_int),
         "glacx_synchronize": ([_vp], C.c_int),
@@ -303,6 +304,18 @@ class Lattice:
         self._check(self._L.glacx_train_grad(self._h, _fptr(smp), n, _fptr(zz), draws, beta, C.byref(loss),
                                              _fptr(grad)), "train_grad")
         return loss.value, grad
+
+    def proposal_eval(self, features):
+        """The sweeps' network on rows of six staple features: (outputs
+        (n, 36), in-range flags (n,))."""
+        f = np.ascontiguousarray(features, dtype=np.float32)
+        n = f.shape[0]
+        assert f.shape == (n, 6)
+        out = np.empty((n, 36), np.float32)
+        ok = np.empty(n, np.int32)
+        self._check(self._L.glacx_proposal_eval(self._h, _fptr(f), n, _fptr(out),
+                                                ok.ctypes.data_as(C.POINTER(C.c_int))), "proposal_eval")
+        return out, ok.astype(bool)
 
     def metropolis_learned(self, n_sweeps, beta, eps=0.24, n_hit=10, reunit_every=1, seed=1, sweep_offset=0,
                            plaquette=False):

@@ -150,18 +150,26 @@ int timed_end(glacx_ctx *c, int kernel, hipEvent_t start)
 // ---- kernel sequences ----------------------------------------------------
 // mode: 0 plain clover observables, 1 with the Weinberg operator, 2 with
 // per-site densities written to `density` (2V doubles).
+// `partials` holds measure_partials_count(geom) doubles.
+template <class T, int R>
+int run_measure_into(glacx_ctx *c, const void *f, double *partials, double *out, int mode, double *density = nullptr)
+{
+    hipEvent_t e0 = nullptr;
+    int rc;
+    if ((rc = timed_begin(c, &e0))) return rc;
+    launch_measure<T, R>(f, c->geom, partials, mode, density, c->stream);
+    CHECK_LAUNCH(c);
+    if ((rc = timed_end(c, GLACX_K_MEASURE, e0))) return rc;
+    launch_measure_finish(partials, c->geom, out, mode == 1, c->stream);
+    CHECK_LAUNCH(c);
+    return GLACX_OK;
+}
+
 template <class T, int R> int run_measure(glacx_ctx *c, const void *f, double *out, int mode, double *density = nullptr)
 {
     int rc = ensure(c, &c->partials, &c->partials_len, (size_t)measure_partials_count(c->geom));
     if (rc) return rc;
-    hipEvent_t e0 = nullptr;
-    if ((rc = timed_begin(c, &e0))) return rc;
-    launch_measure<T, R>(f, c->geom, c->partials, mode, density, c->stream);
-    CHECK_LAUNCH(c);
-    if ((rc = timed_end(c, GLACX_K_MEASURE, e0))) return rc;
-    launch_measure_finish(c->partials, c->geom, out, mode == 1, c->stream);
-    CHECK_LAUNCH(c);
-    return GLACX_OK;
+    return run_measure_into<T, R>(c, f, c->partials, out, mode, density);
 }
 
 template <class T, int R> int run_plaquette(glacx_ctx *c, const void *f, double *out)
@@ -198,31 +206,34 @@ template <class T, int R> int run_flow_step(glacx_ctx *c, void *f, double eps, d
 }
 
 // n_steps flow steps of `field`, measuring before the first and after every
-// step into results (rows of `row` doubles) when results is set.  FP32 links
-// measure inside each step's stage 0 (the field it reads is the previous
-// step's result) and once more after the last step; FP64 links, the
+// step into results (rows of `row` doubles) when results is set, with
+// `partials` (measure_partials_count(geom) doubles) as the reduction buffer.
+// Each step's stage 0 measures the field it reads (the previous step's
+// result), and the clover kernel measures once more after the last step; the
 // SuperSampler set and lattices whose runs are not whole 32-site chunks use
 // the clover kernel between steps.
 template <class T, int R>
-int run_flow(glacx_ctx *c, int field, int n_steps, double eps, double *results, size_t row, bool super)
+int run_flow(glacx_ctx *c, int field, int n_steps, double eps, double *results, size_t row, bool super,
+             double *partials)
 {
-    const bool fuse = results && !super && sizeof(T) == 4 && flow_measure_fusable(c->geom);
+    const bool fuse = results && !super && flow_measure_fusable(c->geom);
     const int mode = super ? 1 : 0;
     int rc;
-    if (fuse && (rc = ensure(c, &c->partials, &c->partials_len, (size_t)measure_partials_count(c->geom)))) return rc;
-    if (results && !fuse && (rc = run_measure<T, R>(c, c->field[field], results, mode))) return rc;
+    if (results && !fuse && (rc = run_measure_into<T, R>(c, c->field[field], partials, results, mode))) return rc;
     for (int i = 0; i < n_steps; i++) {
-        if ((rc = run_flow_step<T, R>(c, c->field[field], eps, fuse ? c->partials : nullptr))) return rc;
+        if ((rc = run_flow_step<T, R>(c, c->field[field], eps, fuse ? partials : nullptr))) return rc;
         if (fuse) {
-            launch_measure_finish(c->partials, c->geom, results + row * (size_t)i, false, c->stream);
+            launch_measure_finish(partials, c->geom, results + row * (size_t)i, false, c->stream);
             CHECK_LAUNCH(c);
         }
         // three stages leave the result in the work buffer: swap roles
         std::swap(c->field[field], c->work);
-        if (results && !fuse && (rc = run_measure<T, R>(c, c->field[field], results + row * (size_t)(i + 1), mode)))
+        if (results && !fuse &&
+            (rc = run_measure_into<T, R>(c, c->field[field], partials, results + row * (size_t)(i + 1), mode)))
             return rc;
     }
-    if (fuse && (rc = run_measure<T, R>(c, c->field[field], results + row * (size_t)n_steps, mode))) return rc;
+    if (fuse && (rc = run_measure_into<T, R>(c, c->field[field], partials, results + row * (size_t)n_steps, mode)))
+        return rc;
     return GLACX_OK;
 }
 
@@ -666,7 +677,9 @@ int flow_impl(glacx_ctx *c, int field, int n_steps, double eps, bool super, doub
     const size_t row = super ? (size_t)4 + 2 * nt : (size_t)3 + nt;
     int rc;
     if (meas && (rc = ensure(c, &c->results, &c->results_len, row * (size_t)(n_steps + 1)))) return rc;
-    if ((rc = DISPATCH(c, run_flow, c, field, n_steps, eps, meas ? c->results : nullptr, row, super))) return rc;
+    if (meas && (rc = ensure(c, &c->partials, &c->partials_len, (size_t)measure_partials_count(c->geom)))) return rc;
+    if ((rc = DISPATCH(c, run_flow, c, field, n_steps, eps, meas ? c->results : nullptr, row, super, c->partials)))
+        return rc;
     if (meas) {
         std::vector<double> h(row * (size_t)(n_steps + 1));
         HIP_TRY(c, hipMemcpyAsync(h.data(), c->results, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -884,6 +897,33 @@ int glacx_train_grad(glacx_ctx *c, const float *samples, int n, const float *z, 
     (void)hipStreamSynchronize(c->stream);
     (void)hipFree(dsmp);
     (void)hipFree(dz);
+    return rc;
+}
+
+int glacx_proposal_eval(glacx_ctx *c, const float *features, int n, float *out, int *in_range)
+{
+    CTX_CHECK(c);
+    if (!c->proposal_ready) return fail(c, GLACX_E_STATE, "proposal_eval: set parameters first");
+    if (!features || !out || n < 1) return fail(c, GLACX_E_ARG, "proposal_eval: bad arguments");
+    const size_t nf = (size_t)6 * n, no = (size_t)36 * n;
+    void *buf = nullptr;
+    HIP_TRY(c, hipMalloc(&buf, (nf + no) * sizeof(float) + (size_t)n * sizeof(int)));
+    float *dfeat = (float *)buf, *dout = dfeat + nf;
+    int *dok = (int *)(dout + no);
+    int rc = GLACX_OK;
+    if (hipMemcpyAsync(dfeat, features, nf * sizeof(float), hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+        rc = fail(c, GLACX_E_HIP, "proposal_eval: copy failed");
+    } else {
+        launch_proposal_eval(dfeat, n, c->params_bf16, c->mean_std, dout, dok, c->stream);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(out, dout, no * sizeof(float), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            (in_range && hipMemcpyAsync(in_range, dok, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, c->stream) !=
+                             hipSuccess) ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = fail(c, GLACX_E_HIP, "proposal_eval: launch or readback failed");
+    }
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(buf);
     return rc;
 }
 

@@ -109,6 +109,12 @@ __device__ __forceinline__ su3alg<double> acc_load(const double *acc, const Geom
 // and the plaquette sum takes Re tr L1 of each plane from its first-index
 // wavefront.  The block's sums go to the clover kernel's (chunk, t) slot.
 
+// With FP64 links the F slots (24.6 KB per block) and the eight shared links
+// (73.7 KB) would leave room for one block per CU; the FP64 measuring stage
+// therefore shares only U_mu(x) through LDS and gathers the backward links
+// U_nu(x - nu) from memory (four more gathers per link), which keeps it at
+// two blocks per CU.
+
 // Slot of GLAC's plane {d1, d2} stored as (d1, d2): 0..2 = (0, nu), 3 = (2,3),
 // 4 = (3,1), 5 = (1,2); (mu, nu) is stored as such when mu = 0 or nu = mu % 3 + 1.
 __device__ __forceinline__ bool plane_first(int mu, int nu) { return mu == 0 || nu == mu % 3 + 1; }
@@ -127,6 +133,20 @@ template <bool DA, class T> __device__ __forceinline__ void ta2_product(const M3
 {
     M3<T> m;
     mm<DA, false, false>(m, a, b);
+    const T third = (m.e[0][0].i + m.e[1][1].i + m.e[2][2].i) * T(2.0 / 3.0);
+    f[0] = T(2) * m.e[0][0].i - third;
+    f[1] = T(2) * m.e[1][1].i - third;
+    f[2] = m.e[0][1].r - m.e[1][0].r;
+    f[3] = m.e[0][1].i + m.e[1][0].i;
+    f[4] = m.e[0][2].r - m.e[2][0].r;
+    f[5] = m.e[0][2].i + m.e[2][0].i;
+    f[6] = m.e[1][2].r - m.e[2][1].r;
+    f[7] = m.e[1][2].i + m.e[2][1].i;
+}
+
+// T(M) in the same form, from the matrix itself.
+template <class T> __device__ __forceinline__ void ta2_of(const M3<T> &m, T f[8])
+{
     const T third = (m.e[0][0].i + m.e[1][1].i + m.e[2][2].i) * T(2.0 / 3.0);
     f[0] = T(2) * m.e[0][0].i - third;
     f[1] = T(2) * m.e[1][1].i - third;
@@ -192,8 +212,10 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : 2) void f
                                                             T *__restrict__ acc, Geom g, T cz, T ca,
                                                             double *__restrict__ meas)
 {
-    static_assert(!MEAS || (STAGE == 0 && sizeof(T) == 4), "measurement is fused into FP32 stage 0 only");
-    __shared__ FlowLds<T> sh;
+    static_assert(!MEAS || STAGE == 0, "the measurement is fused into stage 0 only");
+    // backward links U_nu(x - nu) shared through LDS, except in the FP64 measuring stage
+    constexpr bool BACK_LDS = !(MEAS && sizeof(T) == 8);
+    __shared__ LdsMats<T, BACK_LDS ? 8 : 4> sh;
     const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // one direction per wavefront
     const int lane = threadIdx.x & 63;
     const int hb = flow_chunk(g, xcd_block(blockIdx.x, gridDim.x));
@@ -211,9 +233,16 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : 2) void f
     nbr[0][mu][lane] = ipm;
     nbr[1][mu][lane] = imm;
     lds_put(sh, mu, lane, load_link<T, R>(vmu, idx));
-    lds_put(sh, 4 + mu, lane, load_link<T, R>(vmu, imm));
+    if constexpr (BACK_LDS) lds_put(sh, 4 + mu, lane, load_link<T, R>(vmu, imm));
     T plaq = T(0);
     __syncthreads();
+    // U_d(x - d), from LDS or gathered (x - d from the wavefront of d)
+    auto back_link = [&](int d) -> M3<T> {
+        if constexpr (BACK_LDS)
+            return lds_get(sh, 4 + d, lane);
+        else
+            return load_link<T, R>(dir_view<T>(win, g, d), nbr[1][d][lane]);
+    };
     // staple A_mu(x) = sum_{nu != mu} [ U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
     //                                 + U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu) ]
     // (GLAC/actions/wilsongaugeaction.cpp:29-53)
@@ -229,6 +258,10 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : 2) void f
             if constexpr (MEAS) {
                 // the same staple terms, kept apart for the clover leaves
                 M3<T> up, low;
+                const bool first = plane_first(mu, nu);
+                const int d1 = first ? mu : nu, d2 = first ? nu : mu;
+                const int slot = plane_slot(d1, d2);
+                T f[8];
                 if constexpr (R == 2) {
                     // SU(3) links (checked at upload): products from two rows,
                     // third rows rebuilt
@@ -237,42 +270,59 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : 2) void f
                     mm<false, true, false, 2>(up, p, lds_get(sh, nu, lane));
                     rebuild_third_row(up);
                     const M3<T> q = su3_product<false>(load_rows01<T, R>(vmu, imn), load_link<T, R>(vnu, ipmmn));
-                    mm<true, false, false>(low, q, lds_get(sh, 4 + nu, lane));
+                    mm<true, false, false>(low, q, back_link(nu));
+                    add_to(a, up);
+                    add_to(a, low);
+                    const M3<T> u = lds_get(sh, mu, lane);
+                    if (first) {
+                        plaq += retrace_mul(u, up);
+                        sub_from(up, low);
+                        ta2_product<false>(u, up, f);
+                    } else {
+                        ta2_product<false>(u, low, f);
+                    }
                 } else {
-                    // three stored rows may hold GLAC's det = -1 links: full products
+                    // three stored rows may hold GLAC's det = -1 links: full
+                    // products.  The upper term is finished (its leaf's
+                    // plaquette and T(U_mu up)) before the lower one is
+                    // formed, which keeps fewer matrices live: T(U_mu (up -
+                    // low)) = T(U_mu up) - T(U_mu low).
                     mm<false, true, false>(up, mul_ad(load_link<T, R>(vnu, ipm), load_link<T, R>(vmu, ipn)),
                                            lds_get(sh, nu, lane));
+                    add_to(a, up);
+#pragma unroll
+                    for (int j = 0; j < 8; j++) f[j] = T(0);
+                    if (first) {
+                        const M3<T> u = lds_get(sh, mu, lane);
+                        plaq += retrace_mul(u, up);
+                        ta2_product<false>(u, up, f);
+                    }
                     mm<false, false, false>(low, mul_dd(load_link<T, R>(vnu, ipmmn), load_link<T, R>(vmu, imn)),
-                                            lds_get(sh, 4 + nu, lane));
-                }
-                add_to(a, up);
-                add_to(a, low);
-                const M3<T> u = lds_get(sh, mu, lane);
-                const bool first = plane_first(mu, nu);
-                const int d1 = first ? mu : nu, d2 = first ? nu : mu;
-                const int slot = plane_slot(d1, d2);
-                T f[8];
-                if (first) {
-                    plaq += retrace_mul(u, up);
-                    sub_from(up, low);
-                    ta2_product<false>(u, up, f);
-                } else {
-                    ta2_product<false>(u, low, f);
+                                            back_link(nu));
+                    add_to(a, low);
+                    T fl[8];
+                    ta2_product<false>(lds_get(sh, mu, lane), low, fl);
+#pragma unroll
+                    for (int j = 0; j < 8; j++) f[j] = first ? f[j] - fl[j] : fl[j];
                 }
                 if ((kPhaseL3 >> e) & 1) {
                     // L3 = (U_d2(y) U_d1(x-d1))^dag (U_d1(y) U_d2(x-d2)),  y = x - d1 - d2
                     const int iy = site_step(g, site_step(g, x, d1, -1), d2, -1).idx;
                     const DirView<T> v1 = first ? vmu : vnu, v2 = first ? vnu : vmu;
-                    M3<T> m1, m2;
-                    if constexpr (R == 2) {
-                        m1 = su3_product<false>(load_rows01<T, R>(v1, iy), lds_get(sh, 4 + d2, lane));
-                        m2 = su3_product<false>(load_rows01<T, R>(v2, iy), lds_get(sh, 4 + d1, lane));
-                    } else {
-                        m1 = mul(load_link<T, R>(v1, iy), lds_get(sh, 4 + d2, lane));
-                        m2 = mul(load_link<T, R>(v2, iy), lds_get(sh, 4 + d1, lane));
-                    }
                     T f3[8];
-                    ta2_product<true>(m2, m1, f3);
+                    if constexpr (R == 2) {
+                        const M3<T> m1 = su3_product<false>(load_rows01<T, R>(v1, iy), back_link(d2));
+                        const M3<T> m2 = su3_product<false>(load_rows01<T, R>(v2, iy), back_link(d1));
+                        ta2_product<true>(m2, m1, f3);
+                    } else {
+                        // L3 as a chain U_d1(x-d1)^dag U_d2(y)^dag U_d1(y) U_d2(x-d2):
+                        // the same three products with one matrix live between them
+                        M3<T> c1, c2;
+                        mm<true, true, false>(c1, back_link(d1), load_link<T, R>(v2, iy));
+                        mm<false, false, false>(c2, c1, load_link<T, R>(v1, iy));
+                        mm<false, false, false>(c1, c2, back_link(d2));
+                        ta2_of(c1, f3);
+                    }
 #pragma unroll
                     for (int j = 0; j < 8; j++) f[j] += f3[j];
                 }
@@ -407,12 +457,10 @@ void launch_flow_stage(const void *win, void *wout, void *acc, const Geom &g, in
     T *a = (T *)acc;
     switch (stage) {
     case 0:
-        if constexpr (sizeof(T) == 4) {
-            if (meas) {
-                flow_stage_kernel<T, R, 0, true>
-                    <<<blocks, kBlock, GLACX_FLOW_LDS_BYTES, s>>>(in, out, a, g, T(0.25 * eps), T(0), meas);
-                break;
-            }
+        if (meas) {
+            flow_stage_kernel<T, R, 0, true>
+                <<<blocks, kBlock, GLACX_FLOW_LDS_BYTES, s>>>(in, out, a, g, T(0.25 * eps), T(0), meas);
+            break;
         }
         flow_stage_kernel<T, R, 0, false>
             <<<blocks, kBlock, GLACX_FLOW_LDS_BYTES, s>>>(in, out, a, g, T(0.25 * eps), T(0), nullptr);

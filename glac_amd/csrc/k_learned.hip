@@ -386,7 +386,39 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 3 : 1) void metropolis_lea
     if (threadIdx.x == 0) atomicAdd(count, (unsigned long long)nacc[0]);
 }
 
+// The sampling kernels' matrix-core forward pass on n given feature rows (6
+// floats each): 36 outputs per row into out, and whether the row's
+// standardised features are in range into ok.  A check of the network the
+// sweeps evaluate; idle lanes of the last block mirror row n - 1.
+__global__ __launch_bounds__(kBlock) void proposal_eval_kernel(const float *__restrict__ feat, int n,
+                                                               const unsigned short *__restrict__ wbf16,
+                                                               const float *__restrict__ mean_std,
+                                                               float *__restrict__ out, int *__restrict__ ok)
+{
+    __shared__ __align__(16) MlpImage im;
+    __shared__ __align__(16) float obuf[kBlock / 64][64 * kMlpOut];
+    mlp_image_fill(im, wbf16, threadIdx.x, kBlock);
+    __syncthreads();
+    const int i_raw = blockIdx.x * kBlock + threadIdx.x;
+    const int i = i_raw < n ? i_raw : n - 1;
+    float fe[kMlpIn], o[kMlpOut];
+#pragma unroll
+    for (int k = 0; k < kMlpIn; k++) fe[k] = feat[(size_t)kMlpIn * i + k];
+    const bool in = mlp_forward_mfma(im, mean_std, mean_std + kMlpIn, fe, obuf[threadIdx.x >> 6], o);
+    if (i_raw < n) {
+#pragma unroll
+        for (int k = 0; k < kMlpOut; k++) out[(size_t)kMlpOut * i + k] = o[k];
+        ok[i] = in ? 1 : 0;
+    }
+}
+
 // ---- launchers -------------------------------------------------------------------
+void launch_proposal_eval(const float *feat, int n, const unsigned short *wbf16, const float *mean_std, float *out,
+                          int *ok, hipStream_t s)
+{
+    proposal_eval_kernel<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(feat, n, wbf16, mean_std, out, ok);
+}
+
 template <class T, int R>
 void launch_collect(const void *f, const Geom &g, int mu, int parity, int stride, float *out, size_t cap, size_t base,
                     hipStream_t s)

@@ -44,49 +44,50 @@ __device__ __forceinline__ M3<T> su3_chain4(const M3<T> &a, const M3<T> &b, cons
 }
 
 template <class T, int R>
-__device__ __forceinline__ M3<T> clover(const typename vec2<T>::type *__restrict__ f, const Geom &g, const Site &x,
+__device__ __forceinline__ M3<T> clover(const typename vec2<T>::type *__restrict__ f, const Geom &g, const Coord &x,
                                         int mu, int nu, T &plaq)
 {
-    const Site xpm = site_step(g, x, mu, +1), xmm = site_step(g, x, mu, -1);
-    const int ix = x.idx, imn = site_step(g, x, nu, -1).idx, imm = xmm.idx;
-    const int ipm = xpm.idx, ipn = site_step(g, x, nu, +1).idx, ipmmn = site_step(g, xpm, nu, -1).idx;
-    const int immmn = site_step(g, xmm, nu, -1).idx, immpn = site_step(g, xmm, nu, +1).idx;
+    const Coord xpm = step(g, x, mu, +1), xpn = step(g, x, nu, +1);
+    const Coord xmm = step(g, x, mu, -1), xmn = step(g, x, nu, -1);
+    const Coord xpmmn = step(g, xpm, nu, -1), xmmmn = step(g, xmm, nu, -1), xmmpn = step(g, xmm, nu, +1);
+    const int ix = index_of(g, x), imn = index_of(g, xmn), imm = index_of(g, xmm);
     const DirView<T> vmu = dir_view<T>(f, g, mu), vnu = dir_view<T>(f, g, nu);
     const M3<T> unu = load_link<T, R>(vnu, ix);
+    const int immmn = index_of(g, xmmmn);
     if constexpr (R == 2) {
         const M3<T> unu_mn = load_link<T, R>(vnu, imn);
         const M3<T> umu_mm = load_link<T, R>(vmu, imm);
         const M3<T> umu = load_rows01<T, R>(vmu, ix);  // a leading factor only: two rows
         // L1 = U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
-        M3<T> c = su3_chain4<false, false, true, true, R>(umu, load_link<T, R>(vnu, ipm),
-                                                          load_link<T, R>(vmu, ipn), unu);
+        M3<T> c = su3_chain4<false, false, true, true, R>(umu, load_link<T, R>(vnu, index_of(g, xpm)),
+                                                          load_link<T, R>(vmu, index_of(g, xpn)), unu);
         plaq = retrace(c);
         // - L2 = -U_mu(x) U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu)
-        sub_from(c, su3_chain4<false, true, true, false, R>(umu, load_link<T, R>(vnu, ipmmn),
+        sub_from(c, su3_chain4<false, true, true, false, R>(umu, load_link<T, R>(vnu, index_of(g, xpmmn)),
                                                             load_link<T, R>(vmu, imn), unu_mn));
         // + L3 = U_mu(x-mu)^dag U_nu(x-mu-nu)^dag U_mu(x-mu-nu) U_nu(x-nu)
         add_to(c, su3_chain4<true, true, false, false, R>(umu_mm, load_link<T, R>(vnu, immmn),
                                                           load_link<T, R>(vmu, immmn), unu_mn));
         // - L4 = -U_mu(x-mu)^dag U_nu(x-mu) U_mu(x-mu+nu) U_nu(x)^dag
         sub_from(c, su3_chain4<true, false, false, true, R>(umu_mm, load_link<T, R>(vnu, imm),
-                                                            load_link<T, R>(vmu, immpn), unu));
+                                                            load_link<T, R>(vmu, index_of(g, xmmpn)), unu));
         return c;
     } else {
         // full products, the last one of each leaf accumulated into C
         const M3<T> umu = load_link<T, R>(vmu, ix);
-        M3<T> c = mul_ad(mul_ad(mul(umu, load_link<T, R>(vnu, ipm)), load_link<T, R>(vmu, ipn)),
+        M3<T> c = mul_ad(mul_ad(mul(umu, load_link<T, R>(vnu, index_of(g, xpm))), load_link<T, R>(vmu, index_of(g, xpn))),
                          unu);
         plaq = retrace(c);
         const M3<T> unu_mn = load_link<T, R>(vnu, imn);
         {
-            M3<T> m = mul_ad(mul_ad(umu, load_link<T, R>(vnu, ipmmn)), load_link<T, R>(vmu, imn));
+            M3<T> m = mul_ad(mul_ad(umu, load_link<T, R>(vnu, index_of(g, xpmmn))), load_link<T, R>(vmu, imn));
             negate(m);
             mm<false, false, true>(c, m, unu_mn);
         }
         const M3<T> umu_mm = load_link<T, R>(vmu, imm);
         mm<false, false, true>(c, mul(mul_dd(umu_mm, load_link<T, R>(vnu, immmn)), load_link<T, R>(vmu, immmn)), unu_mn);
         {
-            M3<T> m = mul(mul_da(umu_mm, load_link<T, R>(vnu, imm)), load_link<T, R>(vmu, immpn));
+            M3<T> m = mul(mul_da(umu_mm, load_link<T, R>(vnu, imm)), load_link<T, R>(vmu, index_of(g, xmmpn)));
             negate(m);
             mm<false, true, true>(c, m, unu);
         }
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(kMeasSites * kMeasWaves, sizeof(T) == 8 ? 2 : 1) vo
     M3<T> c2;
     int lex = 0;
     if (valid) {
-        const Site x = site_of(g, run_index(g, blk, lane >> 5, lt, k));
+        const Coord x = decode(g, run_index(g, blk, lane >> 5, lt, k));
         lex = lex_of(g, x);
         const int rho = nu % 3 + 1, sigma = rho % 3 + 1;
         T p1, p2;

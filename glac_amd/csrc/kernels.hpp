@@ -118,6 +118,8 @@ template <class T, int R>
 void launch_metropolis_learned(void *f, const Geom &g, int mu, int parity, const MetroParams &p,
                                const unsigned short *wbf16, const float *mean_std, unsigned long long *count,
                                hipStream_t s);
+void launch_proposal_eval(const float *feat, int n, const unsigned short *wbf16, const float *mean_std, float *out,
+                          int *ok, hipStream_t s);
 constexpr int kTrainInterCols = 170;  // per-sample columns kept by the training step
 constexpr int kProposalParams = 2468;
 

@@ -156,11 +156,11 @@ int glacx_plaquette(glacx_ctx *ctx, int field, double *plaq);
  * non-NULL the clover observables are measured before the first step and
  * after every step, as System::flowConfiguration does
  * (GLAC/system.cpp:471-517): plaq, e_raw and q hold n_steps + 1 values and
- * q_t holds (n_steps + 1) * Nt values, row i = flow time i * eps.  With FP32
- * links, when b_z Ny Nx / 2 is a multiple of 32 (b_z = 4 if 4 divides Nz,
- * else 2; so Nx Ny divisible by 16, or by 32 when 4 does not divide Nz),
- * each step's first RK3 stage measures the field it reads, so a measured
- * flow costs little more than an unmeasured one. */
+ * q_t holds (n_steps + 1) * Nt values, row i = flow time i * eps.  When
+ * b_z Ny Nx / 2 is a multiple of 32 (b_z = 4 if 4 divides Nz, else 2; so
+ * Nx Ny divisible by 16, or by 32 when 4 does not divide Nz), each step's
+ * first RK3 stage measures the field it reads, so a measured flow costs
+ * little more than an unmeasured one. */
 int glacx_flow(glacx_ctx *ctx, int field, int n_steps, double eps,
                double *plaq, double *e_raw, double *q, double *q_t);
 
@@ -205,6 +205,12 @@ int glacx_train(glacx_ctx *ctx, int steps, int batch, int draws, double lr, doub
  * update, for checking. */
 int glacx_train_grad(glacx_ctx *ctx, const float *samples, int n, const float *z, int draws, double beta,
                      double *loss, float *grad);
+/* The network as the learned sweeps evaluate it (bf16 matrix-core GEMMs,
+ * FP32 accumulation) on n host rows of the six staple features: 36 outputs
+ * per row into out (n x 36 floats), and in_range[i] (may be NULL) = 1 when
+ * row i's standardised features lie in [-8, 8] (else the sweep falls back to
+ * the fixed-eps proposal).  For checking, like glacx_train_grad. */
+int glacx_proposal_eval(glacx_ctx *ctx, const float *features, int n, float *out, int *in_range);
 /* glacx_metropolis with the learned proposal (fixed-eps RST, step eps, for
  * links whose standardised features fall outside [-8, 8]). */
 int glacx_metropolis_learned(glacx_ctx *ctx, int n_sweeps, double beta, double eps, int n_hit, int reunit_every,

@@ -64,6 +64,17 @@ def test_learned_sweep_follows_oracle_streams(prec):
     rng = np.random.default_rng(11)
     p = T.init_params(rng, scale0=0.3)
     p[T.SLICES["W3"]] += rng.normal(size=36 * 32) * 0.02
+    # The network's layer inputs are rounded to bf16.  An input that the FP32
+    # device and the FP64 oracle round to neighbouring bf16 values changes a
+    # proposal by ~1e-3, that link's staple-dependent neighbours' features
+    # with it, and so on through the sweep (a 1e-7 perturbation of the
+    # parameters changes a quarter of the links of the oracle's own sweep).
+    # With W1 = 0 the network's output is the same for every link, so the
+    # sweep tests the draws, the Cholesky factor, the algebra map, the
+    # exponential, the RST fallback and the accept/reject decisions without
+    # that amplification; test_proposal_network_matches_oracle covers the
+    # feature-dependent network.
+    p[T.SLICES["W1"]] = 0.0
     pb = bf16_round(p)
     U = O.hot_start_haar(n, 3)
     for s in range(3):
@@ -80,12 +91,52 @@ def test_learned_sweep_follows_oracle_streams(prec):
         L.proposal_set(p, mean, std)
         got_acc = L.metropolis_learned(1, 6.0, 0.24, 10, reunit_every=1, seed=21)
         got = L.download()
-    nlinks = 4 * 10 * np.prod(n)
-    # the network runs in FP32 on the device (FP64 in the oracle): allow a
-    # handful of flipped near-threshold decisions
-    assert abs(acc - got_acc[0]) * nlinks <= 3
+    decisions = 10 * 4 * np.prod(n)
+    # The network accumulates in FP32 on the device (FP64 in the oracle), so
+    # a decision whose exp(-dS) lies within ~1e-7 of its uniform could flip.
     close = np.abs(got - W).reshape(4, -1, 18).max(axis=-1) < (1e-5 if prec == glacx.FP64 else 1e-4)
-    assert close.mean() > 0.99
+    n_diff = int((~close).sum())
+    print(f"links differing from the oracle: {n_diff} of {close.size}; "
+          f"accepted hits {acc * decisions:.0f} (oracle) vs {got_acc[0] * decisions:.0f}")
+    assert n_diff <= 2
+    assert abs(acc - got_acc[0]) * decisions <= 3
+
+
+def test_proposal_network_matches_oracle():
+    """The sweeps' matrix-core network (bf16 GEMMs, FP32 accumulation) on
+    staple features of a thermalised field against the oracle's FP64
+    evaluation with the same bf16 rounding of every layer input.  Where the
+    two accumulations round a hidden activation to neighbouring bf16 values
+    the output moves by ~1e-3 of its scale: such rows are rare, the rest agree
+    to FP32 accumulation error."""
+    banner("learned: network outputs vs oracle")
+    rng = np.random.default_rng(13)
+    n = (4, 4, 4, 8)
+    U = O.hot_start_haar(n, 5)
+    for s in range(5):
+        O.cb_sweep(U, n, 6.0, 0.24, 10, 5, s)
+    f = np.array([O.staple_features(O.staple(U, n, x, y, z, t, mu), 6.0)
+                  for x in range(4) for y in range(4) for z in range(4) for t in range(8) for mu in range(4)])
+    mean, std = f.mean(0).astype(np.float32), (f.std(0) + 1e-3).astype(np.float32)
+    f32 = f.astype(np.float32)
+    f32[:8] = mean + 9.0 * std * np.sign(rng.normal(size=(8, 6)))   # out of range: fallback rows
+    p = T.init_params(rng, scale0=0.3)
+    p[T.SLICES["W3"]] += rng.normal(size=36 * 32) * 0.05
+    pb = bf16_round(p)
+    with glacx.Lattice((4, 4, 4, 4), glacx.FP32_LINKS) as L:
+        L.proposal_set(p, mean, std)
+        out, ok = L.proposal_eval(f32)
+    want = np.array([O.mlp_forward(pb.astype(np.float64), mean.astype(np.float64), std.astype(np.float64),
+                                   row.astype(np.float64)) for row in f32])
+    x = (f32.astype(np.float64) - mean) / std
+    assert np.array_equal(ok, np.all(np.abs(x) <= 8.0, axis=1))
+    assert not ok[:8].any() and ok[8:].mean() > 0.99
+    err = np.abs(out - want).max(axis=1) / np.abs(want).max()
+    print(f"network rows {len(err)}: median error {np.median(err):.2e}, 99th percentile "
+          f"{np.quantile(err, 0.99):.2e}, max {err.max():.2e}")
+    assert np.median(err) < 1e-6
+    assert np.mean(err < 1e-5) > 0.97
+    assert err.max() < 3e-2
 
 
 def test_pipeline_trains_and_samples_the_right_distribution():

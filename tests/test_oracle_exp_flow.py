@@ -85,6 +85,32 @@ def test_flow_is_third_order():
     assert 6.0 < ratio < 10.5, (errs, ratio)  # 2^3 = 8 for a third-order scheme
 
 
+def test_flow_time_scale_matches_an_independent_integrator():
+    """The RK3 scheme integrates dV/dt = Z(V) V over exactly eps per step
+    (GLAC/flow/flow.cpp:43-73): four steps of 0.0125 against 2000 Lie-Euler
+    steps V <- exp(dt Z(V)) V of the same ODE, with Z from the oracle's
+    action derivative and the exponential from its eigendecomposition.  The
+    third-order test alone leaves the scale open (a flow of 2 eps per step, or
+    of -eps, converges at the same order); here such a flow misses by ~0.1."""
+    rng = np.random.default_rng(24)
+    U0 = near_unit_field(N4, rng, 0.6)
+    T, n_rk, n_euler = 0.05, 4, 2000
+    U = flowed(U0, N4, T / n_rk, n_rk)
+    V = O.to_complex(U0).reshape(4, -1, 3, 3)
+    dt = T / n_euler
+    for _ in range(n_euler):
+        Z = np.stack([O.to_complex(O.action_derivative(O.to_pairs(V), N4, mu)).reshape(-1, 3, 3)
+                      for mu in range(4)])
+        # exp(dt Z), Z antihermitian: Z = i H with H hermitian
+        w, v = np.linalg.eigh(-1j * dt * Z)
+        V = (v * np.exp(1j * w)[..., None, :]) @ dag(v) @ V
+    err = np.abs(O.to_complex(U).reshape(4, -1, 3, 3) - V).max()
+    wrong = flowed(U0, N4, 2 * T / n_rk, n_rk)
+    err_wrong = np.abs(O.to_complex(wrong).reshape(4, -1, 3, 3) - V).max()
+    assert err < 3e-5, err  # Lie-Euler: 1e-5 at 2000 steps, halving with each doubling
+    assert err_wrong > 100 * err, (err, err_wrong)
+
+
 def test_flow_keeps_pure_gauge_and_unitarity():
     rng = np.random.default_rng(22)
     U = gauge_transform(O.unit_field(N4), N4, random_su3(rng, (4, 4, 4, 4)))
