@@ -75,6 +75,7 @@ def library():
         "glacx_measure": ([_vp, C.c_int, _dp, _dp, _dp, _dp], C.c_int),
         "glacx_plaquette": ([_vp, C.c_int, _dp], C.c_int),
         "glacx_flow": ([_vp, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp], C.c_int),
+        "glacx_flow_noalloc": ([_vp, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp], C.c_int),
         "glacx_action_derivative": ([_vp, C.c_int, C.c_int, _dp], C.c_int),
         "glacx_samples_reset": ([_vp, C.c_size_t], C.c_int),
         "glacx_samples_collect": ([_vp, C.c_int, C.POINTER(C.c_size_t)], C.c_int),
@@ -243,6 +244,19 @@ class Lattice:
         qt = np.empty((n_steps + 1, self.dims[3]))
         self._check(self._L.glacx_flow(self._h, field, n_steps, eps, _ptr(p), _ptr(e), _ptr(q), _ptr(qt)),
                     "flow")
+        return p, e, q, qt
+
+    def flow_noalloc(self, n_steps, eps=0.01, field=FLOW, measure=True):
+        """flow() through glacx_flow_noalloc, which allocates no device memory
+        (its scratch is a fixed per-GPU array)."""
+        if not measure:
+            self._check(self._L.glacx_flow_noalloc(self._h, field, n_steps, eps, None, None, None, None),
+                        "flow_noalloc")
+            return None
+        p, e, q = np.empty(n_steps + 1), np.empty(n_steps + 1), np.empty(n_steps + 1)
+        qt = np.empty((n_steps + 1, self.dims[3]))
+        self._check(self._L.glacx_flow_noalloc(self._h, field, n_steps, eps, _ptr(p), _ptr(e), _ptr(q), _ptr(qt)),
+                    "flow_noalloc")
         return p, e, q, qt
 
     def flow_super(self, n_steps, eps=0.01, field=FLOW):

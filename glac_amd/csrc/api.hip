@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -32,6 +33,7 @@ struct glacx_ctx {
     unsigned long long *counts = nullptr;  // accepted hits per sweep
     size_t counts_len = 0;
     double *stage = nullptr;         // AoS FP64 staging for host transfers
+    double *scratch = nullptr;       // this device's copy of `scratch` (allocation-free calls)
     size_t device_bytes = 0;
     std::string err;
     // per-kernel event timing
@@ -356,6 +358,41 @@ int require_su3(glacx_ctx *c, const double *links, size_t n_links, size_t stride
 
 }  // namespace
 
+// ---- scratch memory of the allocation-free entry point -------------------
+// glacx_flow_noalloc keeps its reduction partials and per-step results in
+// this fixed array instead of context buffers that grow on demand, so the
+// call makes no device allocation.  Every GPU holds its own copy for the
+// life of the process; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (the array's name is a
+// host-side stand-in, not a device address).
+constexpr size_t kScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on c->device (current), looked up once per
+// device; called from glacx_create.
+int scratch_setup(glacx_ctx *c)
+{
+    if (c->device < 0 || c->device >= kMaxDevices)
+        return fail(c, GLACX_E_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_addr[c->device]) {
+        void *p = nullptr;
+        HIP_TRY(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(c, GLACX_E_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+        g_scratch_addr[c->device] = (double *)p;
+    }
+    c->scratch = g_scratch_addr[c->device];
+    return GLACX_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int glacx_create(const int dims[4], int precision, int device, glacx_ctx **out)
@@ -391,6 +428,7 @@ int glacx_create(const int dims[4], int precision, int device, glacx_ctx **out)
     if ((rc = dev_alloc(c, &c->work, field_bytes(c)))) return bad(rc);
     if ((rc = dev_alloc(c, &c->acc, (size_t)32 * v * c->real_size))) return bad(rc);
     if ((rc = ensure(c, &c->results, &c->results_len, (size_t)(3 + dims[3])))) return bad(rc);
+    if ((rc = scratch_setup(c))) return bad(rc);
     *out = c;
     return GLACX_OK;
 }
@@ -713,6 +751,42 @@ int glacx_flow_super(glacx_ctx *c, int field, int n_steps, double eps, double *p
 {
     CTX_CHECK(c);
     return flow_impl(c, field, n_steps, eps, true, plaq, e_raw, q, w, q_t, w_t);
+}
+
+int glacx_flow_noalloc(glacx_ctx *c, int field, int n_steps, double eps, double *plaq, double *e_raw, double *q,
+                       double *q_t)
+{
+    CTX_CHECK(c);
+    if (!valid_field(field) || n_steps < 0) return fail(c, GLACX_E_ARG, "flow_noalloc: bad field or step count");
+    if (!c->scratch) return fail(c, GLACX_E_STATE, "flow_noalloc: no scratch address for this device");
+    const bool meas = plaq || e_raw || q || q_t;
+    const int nt = c->dims[3];
+    const size_t row = (size_t)3 + nt;
+    const size_t np = (size_t)measure_partials_count(c->geom);
+    const size_t need = meas ? np + row * ((size_t)n_steps + 1) : 0;
+    if (need * sizeof(double) > kScratchBytes)
+        return fail(c, GLACX_E_ARG, "flow_noalloc: " + std::to_string(n_steps) + " measured steps on this lattice need " +
+                                        std::to_string(need * sizeof(double)) + " bytes of scratch, above its " +
+                                        std::to_string(kScratchBytes) + "; split the flow into shorter calls");
+    double *partials = c->scratch, *results = c->scratch + np;
+    // event timing would create events: off for this call
+    const bool timing = c->timing;
+    c->timing = false;
+    int rc = DISPATCH(c, run_flow, c, field, n_steps, eps, meas ? results : nullptr, row, false, partials);
+    c->timing = timing;
+    if (rc) return rc;
+    // rows {plaq, E_raw, Q, Q(t)...} to the caller's arrays by strided copies
+    const size_t pitch = row * sizeof(double), rows = (size_t)n_steps + 1;
+    double *cols[3] = {plaq, e_raw, q};
+    for (int k = 0; k < 3; k++)
+        if (cols[k])
+            HIP_TRY(c, hipMemcpy2DAsync(cols[k], sizeof(double), results + k, pitch, sizeof(double), rows,
+                                        hipMemcpyDeviceToHost, c->stream));
+    if (q_t)
+        HIP_TRY(c, hipMemcpy2DAsync(q_t, nt * sizeof(double), results + 3, pitch, nt * sizeof(double), rows,
+                                    hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GLACX_OK;
 }
 
 int glacx_action_derivative(glacx_ctx *c, int field, int mu, double *z_out)

@@ -170,6 +170,20 @@ int glacx_flow(glacx_ctx *ctx, int field, int n_steps, double eps,
 int glacx_flow_super(glacx_ctx *ctx, int field, int n_steps, double eps, double *plaq, double *e_raw, double *q,
                      double *w, double *q_t, double *w_t);
 
+/* glacx_flow that allocates nothing.  No counterpart in GLAC: for callers
+ * that cannot allocate device memory during the call (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * measurement partials and per-step result rows live in a fixed 16 MiB
+ * device array, one per GPU, held until the process ends; the call returns
+ * GLACX_E_ARG when (n_steps + 1) rows of (3 + Nt) doubles plus the clover
+ * reduction's partials (4 Nt (V / (Nt 64) + 1) doubles, about 0.5 MiB at
+ * 32^4) exceed it -- split such a flow into shorter calls.  Calls that
+ * overlap on one GPU (several contexts or threads) share the array and race;
+ * serialise them.  Event timing (glacx_timing_enable) does not record these
+ * calls.  Arguments and results as glacx_flow. */
+int glacx_flow_noalloc(glacx_ctx *ctx, int field, int n_steps, double eps,
+                       double *plaq, double *e_raw, double *q, double *q_t);
+
 /* Whole-lattice flow force for one direction in GLAC's layout, V*18 doubles
  * (WilsonGaugeAction::getActionDerivative,
  * GLAC/actions/wilsongaugeaction.cpp:55-86). */
