@@ -203,6 +203,8 @@ def main():
                          "measurement (learned FP32, RST FP32, RST FP64); 0 skips it")
     ap.add_argument("--only-indep", action="store_true",
                     help="run only the independent-configs/s measurement and print its JSON")
+    ap.add_argument("--indep-variants", default=",".join(VARIANTS),
+                    help="comma-separated variants of the independent-configs/s measurement (%s)" % ", ".join(VARIANTS))
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -218,7 +220,11 @@ def main():
     beta, eps_flow = 6.1, 0.01
 
     if args.only_indep:
-        print(json.dumps(independent_configs(G, n, beta, args.indep_seconds)), flush=True)
+        variants = tuple(v for v in args.indep_variants.split(",") if v)
+        unknown = [v for v in variants if v not in VARIANTS]
+        if unknown:
+            ap.error(f"unknown variants {unknown}")
+        print(json.dumps(independent_configs(G, n, beta, args.indep_seconds, variants)), flush=True)
         return
 
     # ---------------------------------------------------------------- FP32 flow

@@ -244,28 +244,38 @@ __global__ __launch_bounds__(kBlock) void measure_slice_kernel(const double *__r
 // out = {plaq, E_raw, Q, Q(t = 0..Nt-1)}, or with `super`
 // {plaq, E_raw, Q, W, Q(t)..., W(t)...}; W's normalisation is GLAC's
 // 2 * 8 * 8 / (3 * 16^3) (GLAC/observables/supersampler.cpp:17-19).
-__global__ void measure_finish_kernel(const double *__restrict__ slice, Geom g, int super, double *__restrict__ out)
+// One wavefront: lane l takes the slices t = l, l + 64, ... in order, then
+// the lanes' sums are combined by a fixed shuffle tree (the order never
+// depends on scheduling).
+__global__ __launch_bounds__(64) void measure_finish_kernel(const double *__restrict__ slice, Geom g, int super,
+                                                            double *__restrict__ out)
 {
-    if (threadIdx.x != 0) return;
     const double qnorm = 1.0 / (256.0 * M_PI * M_PI), wnorm = 2.0 * 8.0 * 8.0 / (3.0 * 16.0 * 16.0 * 16.0);
-    const int nt = g.n[3], head = super ? 4 : 3;
-    double pl = 0.0, en = 0.0, q = 0.0, w = 0.0;
-    for (int t = 0; t < nt; t++) {
-        pl += slice[kMeasAcc * t + 0];
-        en += slice[kMeasAcc * t + 1];
-        const double qt = slice[kMeasAcc * t + 2] * qnorm;
+    const int nt = g.n[3], head = super ? 4 : 3, lane = threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};  // plaquette, energy, q, w
+    for (int t = lane; t < nt; t += 64) {
+        const double *st = slice + kMeasAcc * t;
+        v[0] += st[0];
+        v[1] += st[1];
+        const double qt = st[2] * qnorm;
         out[head + t] = qt;
-        q += qt;
+        v[2] += qt;
         if (super) {
-            const double wt = slice[kMeasAcc * t + 3] * wnorm;
+            const double wt = st[3] * wnorm;
             out[head + nt + t] = wt;
-            w += wt;
+            v[3] += wt;
         }
     }
-    out[0] = pl / (18.0 * (double)g.v);
-    out[1] = en / (double)g.v;
-    out[2] = q;
-    if (super) out[3] = w;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+    if (lane == 0) {
+        out[0] = v[0] / (18.0 * (double)g.v);
+        out[1] = v[1] / (double)g.v;
+        out[2] = v[2];
+        if (super) out[3] = v[3];
+    }
 }
 
 // Plaquette only: sum over sites and the 6 planes of
