@@ -206,15 +206,24 @@ template <class T> __device__ __forceinline__ FlowMeasLds<T> &flow_meas_lds()
     return s;
 }
 
+// FP64 tuning knobs (build time): waves per SIMD the stage is compiled for,
+// and whether its plain stages share the backward links through LDS.
+#ifndef GLACX_FP64_FLOW_WAVES
+#define GLACX_FP64_FLOW_WAVES 2
+#endif
+#ifndef GLACX_FP64_BACK_LDS
+#define GLACX_FP64_BACK_LDS 1
+#endif
+
 template <class T, int R, int STAGE, bool MEAS>
-__global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : 2) void flow_stage_kernel(const typename vec2<T>::type *__restrict__ win,
+__global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : (MEAS ? 2 : GLACX_FP64_FLOW_WAVES)) void flow_stage_kernel(const typename vec2<T>::type *__restrict__ win,
                                                             typename vec2<T>::type *__restrict__ wout,
                                                             T *__restrict__ acc, Geom g, T cz, T ca,
                                                             double *__restrict__ meas)
 {
     static_assert(!MEAS || STAGE == 0, "the measurement is fused into stage 0 only");
     // backward links U_nu(x - nu) shared through LDS, except in the FP64 measuring stage
-    constexpr bool BACK_LDS = !(MEAS && sizeof(T) == 8);
+    constexpr bool BACK_LDS = !(sizeof(T) == 8 && (MEAS || !GLACX_FP64_BACK_LDS));
     __shared__ LdsMats<T, BACK_LDS ? 8 : 4> sh;
     const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // one direction per wavefront
     const int lane = threadIdx.x & 63;
@@ -350,7 +359,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : 2) void f
                 M3<T> q;  // U_mu(x-nu) U_nu(x+mu-nu), whose dagger the lower staple uses
                 mm<false, false, false, 2>(q, load_rows01<T, R>(vmu, imn), load_link<T, R>(vnu, ipmmn));
                 rebuild_third_row(q);
-                mm<true, false, true>(a, q, lds_get(sh, 4 + nu, lane));
+                mm<true, false, true>(a, q, back_link(nu));
             } else {
                 {
                     const M3<T> u1 = load_link<T, R>(vnu, ipm);
@@ -360,7 +369,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : 2) void f
                 {
                     const M3<T> u1 = load_link<T, R>(vnu, ipmmn);
                     const M3<T> u2 = load_link<T, R>(vmu, imn);
-                    mm<false, false, true>(a, mul_dd(u1, u2), lds_get(sh, 4 + nu, lane));
+                    mm<false, false, true>(a, mul_dd(u1, u2), back_link(nu));
                 }
             }
         }

@@ -327,9 +327,9 @@ public:
 
     // GLAC's System::flowConfiguration loop (GLAC/system.cpp:494-508) in one
     // call: calculate(lattice, 0), then nFlows times {flowField; calculate},
-    // the lattice flowed in place and observations 0..nFlows filled.  With
-    // FP32 links each step's first RK3 stage measures the field it reads,
-    // which saves the separate clover pass of the step-by-step loop.
+    // the lattice flowed in place and observations 0..nFlows filled.  Each
+    // step's first RK3 stage measures the field it reads, which saves the
+    // separate clover pass of the step-by-step loop.
     void flowConfiguration(L *lattice, unsigned int nFlows, double epsilon)
     {
         if (nFlows + 1 > m_obs.size()) fatal("flowConfiguration: more flow steps than observations");
@@ -468,7 +468,7 @@ public:
     }
     // GLAC's System::flowConfiguration loop (GLAC/system.cpp:494-508) in one
     // call: observations 0..nFlows of the lattice flowed in place, each step's
-    // first RK3 stage measuring the field it reads when the links are FP32.
+    // first RK3 stage measuring the field it reads.
     void flowConfiguration(L *lattice, unsigned int nFlows, double epsilon)
     {
         const int f = m_dev->acquire(lattice);
