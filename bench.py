@@ -97,6 +97,7 @@ def valu_issue_frac(prof, launch_ms):
     return prof["waves"] * prof["valu_per_wave"] * VALU_CYCLES / (SIMDS * CLOCK_HZ * launch_ms * 1e-3)
 
 
+SHARED_THERM = 6000  # RST sweeps from the hot start before the variants of independent_configs split
 VARIANTS = {"fp32c_learned": ("FP32_COMPRESSED", "learned"), "fp32c_rst": ("FP32_COMPRESSED", "rst"),
             "fp64_plain": ("FP64", "rst")}
 
@@ -116,29 +117,36 @@ def independent_configs(G, n, beta, seconds, variants=tuple(VARIANTS)):
     network's gain at equal precision) and GLAC's RST proposal with FP64
     links (the plain FP64 path).
 
-    Each variant: Haar hot start, 1500 RST sweeps; the learned variant then
-    collects 20 sweeps of (staple, link) samples and trains its network
-    (3000 Adam steps, not counted in configs/s); then 300 more sweeps of its
-    own update and configurations spaced by NCor = 50 sweeps, each flowed
-    50 RK3 steps of 0.02 (t = 1) and measured, for about `seconds` of wall
-    time.  tau_int(Q^2) by Wolff's Gamma method, in units of the spacing."""
+    Shared start: Haar hot start and SHARED_THERM RST sweeps in compressed
+    FP32 (about ten tau_int(Q^2) of the RST chain, so every variant starts
+    with its topology settled).  Each variant continues from that field with
+    its own streams: the learned variant collects 20 sweeps of (staple, link)
+    samples and trains its network (3000 Adam steps, not counted in
+    configs/s); then 300 sweeps of its own update and configurations spaced by
+    NCor = 50 sweeps, each flowed 50 RK3 steps of 0.02 (t = 1) and measured,
+    for about `seconds` of wall time.  tau_int(Q^2) by Wolff's Gamma method,
+    in units of the spacing."""
     from glac_amd.system import EnsembleConfig, run_ensemble
     from glac_amd.proposal import train_from_chain
     out = {}
+    with G.Lattice(n, G.FP32_COMPRESSED) as L:
+        L.hot_start(31)
+        L.metropolis(SHARED_THERM, beta, 0.24, 10, seed=3)
+        start = L.download()
     for name in variants:
         prec, proposal = getattr(G, VARIANTS[name][0]), VARIANTS[name][1]
+        seed = 9 + list(VARIANTS).index(name)
         L = G.Lattice(n, prec)
-        L.hot_start(31)
-        L.metropolis(1500, beta, 0.24, 10, seed=3)
-        off = 1500
+        L.upload(start)
+        off = SHARED_THERM
         train_s = 0.0
         if proposal == "learned":
             t0 = time.perf_counter()
-            train_from_chain(L, beta, seed=3, sweep_offset=off, collect_sweeps=20)
+            train_from_chain(L, beta, seed=seed, sweep_offset=off, collect_sweeps=20)
             L.synchronize()
             train_s = time.perf_counter() - t0
             off += 20
-        cfg = EnsembleConfig(dims=n, beta=beta, NTherm=300, NCf=1, NCor=50, NFlows=50, flowEpsilon=0.02, seed=9,
+        cfg = EnsembleConfig(dims=n, beta=beta, NTherm=300, NCf=1, NCor=50, NFlows=50, flowEpsilon=0.02, seed=seed,
                              proposal=proposal)
         pilot = run_ensemble(L, cfg, thermalise=True, sweep_offset=off)
         off += cfg.NTherm + cfg.NCor
@@ -162,7 +170,8 @@ def independent_configs(G, n, beta, seconds, variants=tuple(VARIANTS)):
         out["same_precision"] = {"ratio_learned_over_rst_fp32c": r, "ratio_err": e,
                                  "tau_int_q2_ratio_rst_over_learned":
                                      out["fp32c_rst"]["tau_int_q2"] / out["fp32c_learned"]["tau_int_q2"]}
-    out["protocol"] = ("NCor 50 sweeps (10 hits, eps 0.24 fallback/RST), flow 50 x 0.02, Q at t = 1; "
+    out["protocol"] = (f"shared start: hot start + {SHARED_THERM} RST sweeps (FP32); per variant 300 sweeps, then "
+                       "NCor 50 sweeps (10 hits, eps 0.24 fallback/RST), flow 50 x 0.02, Q at t = 1; "
                        "tau_int(Q^2) by Wolff's Gamma method (S = 1.5), errors from its estimate")
     # the same measurement run for several minutes per variant (committed), for comparison
     # with this run's shorter, noisier estimate
