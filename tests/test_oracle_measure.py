@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from fieldtools import abelian_field, gauge_transform, plaquettes, random_field, random_su3, reflect_t
+from fieldtools import abelian_field, dag, gauge_transform, plaquettes, random_field, random_su3, reflect_t
 
 
 def test_unit_field():
@@ -96,6 +96,22 @@ def test_weinberg_vanishes_on_unit_and_abelian_fields():
     U, phi1, phi2 = abelian_field(n, 1, 1)
     _, _, q, w, _, wt = O.measure_super(U, n)
     assert abs(q) > 0.5 and abs(w) < 1e-12 and np.abs(wt).max() < 1e-12
+
+
+def test_weinberg_is_cubic_and_energy_quadratic_in_the_field_strength():
+    """Near the unit field, U = exp(i s H) with H fixed: E ~ s^2 and the
+    Weinberg operator ~ s^3 (three field-strength tensors; a product of two
+    would scale as s^2).  Halving s divides them by 4 and 8."""
+    rng = np.random.default_rng(5)
+    n = (4, 4, 4, 4)
+    V = int(np.prod(n))
+    a = rng.normal(size=(4, V, 3, 3)) + 1j * rng.normal(size=(4, V, 3, 3))
+    h = (a + dag(a)) / 2
+    h -= np.trace(h, axis1=-2, axis2=-1)[..., None, None] * np.eye(3) / 3
+    w, v = np.linalg.eigh(h)
+    r = [O.measure_super(O.to_pairs(v @ (np.exp(1j * s * w)[..., None] * dag(v))), n) for s in (0.02, 0.01)]
+    assert 3.8 < r[0][1] / r[1][1] < 4.2
+    assert 7.5 < r[0][3] / r[1][3] < 8.7, (r[0][3], r[1][3])
 
 
 # ---- GLAC's LatticeActionChargeDensity: per-site densities
