@@ -60,7 +60,8 @@
  *      fields with flux in three planes sharing a direction, where a wrong
  *      orientation of one G_{nu lambda} term would leave a non-zero
  *      commutator trace; gauge invariance; time reflection flips W;
- *      sum_t W(t) = W.  The overall scale 2 * 8 * 8 / (3 * 16^3) is GLAC's
+ *      sum_t W(t) = W; W scales as s^3 for U = exp(i s H), s -> 0 (three
+ *      field-strength factors).  The overall scale 2 * 8 * 8 / (3 * 16^3) is GLAC's
  *      constant (supersampler.cpp:19) and parity unpinned beyond that
  *      reading: GLAC ships no Weinberg reference values.
  *  or_glac_hot_start / or_glac_sweep   GLAC/system.cpp:200-214, 321-361.
