@@ -212,18 +212,23 @@ template <class T> __device__ __forceinline__ M3<T> exp_su3_proposal(const su3al
         const float c1 = 0.5f * (a0 * a0 + a1 * a1 + a2 * a2) + n01 + n02 + n12;
         const float tr = h01r * h12r - h01i * h12i, ti = h01r * h12i + h01i * h12r;
         const float c0 = a0 * a1 * a2 + 2.f * (tr * h02r + ti * h02i) - a0 * n12 - a1 * n02 - a2 * n01;
-        // N terms leave less than ||H||^(N+1) / (N+1)!, ||H||_F^2 = 2 c1, below
-        // FP32 rounding (6e-8) for c1 <= 0.125 (N = 8), 0.5 (11), 2 (16); the
-        // wavefront takes the largest order any of its series lanes needs.
+        // N terms leave about |h|^(N+1) / (N+1)! of the series, h the largest
+        // eigenvalue of H in magnitude; with tr H = 0 and tr H^2 = 2 c1,
+        // |h| <= sqrt(4 c1 / 3).  That is below FP32 rounding (6e-8) for
+        // c1 <= 0.125 (N = 7), 0.5 (9), 1 (11), 2 (13); the wavefront takes
+        // the largest order any of its series lanes needs (trained steps have
+        // c1 ~ 0.4, so a wavefront of 64 usually holds a lane above 0.5).
         float f[3][2];
         if (c1 > 2.0f)
             exp_coefficients_f32(c0, c1, f);
+        else if (__any(c1 > 1.0f))
+            exp_coefficients_series_f32<13>(c0, c1, f);
         else if (__any(c1 > 0.5f))
-            exp_coefficients_series_f32<16>(c0, c1, f);
-        else if (__any(c1 > 0.125f))
             exp_coefficients_series_f32<11>(c0, c1, f);
+        else if (__any(c1 > 0.125f))
+            exp_coefficients_series_f32<9>(c0, c1, f);
         else
-            exp_coefficients_series_f32<8>(c0, c1, f);
+            exp_coefficients_series_f32<7>(c0, c1, f);
         return exp_assemble<float>(expand(y), f[0][0], f[0][1], f[1][0], f[1][1], f[2][0], f[2][1]);
     }
 }
