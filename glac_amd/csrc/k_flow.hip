@@ -233,6 +233,10 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : (MEAS ? 2
     const int h = valid ? h_raw : g.vh - 1;  // idle lanes of a partial block mirror a real site, store nothing
     const int idx = parity_index(g, lane >> 5, h);
     const Site x = site_of(g, idx);
+    // stages 1 and 2 read the link's accumulator now, so that its latency
+    // passes under the staple loop instead of stalling the step after it
+    su3alg<T> cacc;
+    if constexpr (STAGE != 0) cacc = acc_load(acc, g, mu, idx);
     const DirView<T> vmu = dir_view<T>(win, g, mu);
     // the site's neighbours x + mu and x - mu, shared with the other
     // wavefronts, whose staples step the same way in direction mu
@@ -385,7 +389,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : (MEAS ? 2
 #pragma unroll
             for (int k = 0; k < 6; k++) y.p[k] = cz * z.p[k];
         } else {
-            const su3alg<T> c = acc_load(acc, g, mu, idx);
+            const su3alg<T> &c = cacc;
             y.a0 = cz * z.a0 - ca * c.a0;
             y.a1 = cz * z.a1 - ca * c.a1;
 #pragma unroll
