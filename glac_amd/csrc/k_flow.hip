@@ -127,50 +127,6 @@ constexpr unsigned long long kPhaseNu = 0x210130302321ull;
 constexpr unsigned kPhaseAdd = 0x936u;  // entries 1 2 4 5 8 11
 constexpr unsigned kPhaseL3 = 0x3a3u;   // entries 0 1 5 7 8 9
 
-// T(op(A) B) as {Im T00, Im T11, T01, T02, T12} (the real diagonal of the
-// product is not needed and not computed).
-template <bool DA, class T> __device__ __forceinline__ void ta2_product(const M3<T> &a, const M3<T> &b, T f[8])
-{
-    M3<T> m;
-    mm<DA, false, false>(m, a, b);
-    const T third = (m.e[0][0].i + m.e[1][1].i + m.e[2][2].i) * T(2.0 / 3.0);
-    f[0] = T(2) * m.e[0][0].i - third;
-    f[1] = T(2) * m.e[1][1].i - third;
-    f[2] = m.e[0][1].r - m.e[1][0].r;
-    f[3] = m.e[0][1].i + m.e[1][0].i;
-    f[4] = m.e[0][2].r - m.e[2][0].r;
-    f[5] = m.e[0][2].i + m.e[2][0].i;
-    f[6] = m.e[1][2].r - m.e[2][1].r;
-    f[7] = m.e[1][2].i + m.e[2][1].i;
-}
-
-// T(M) in the same form, from the matrix itself.
-template <class T> __device__ __forceinline__ void ta2_of(const M3<T> &m, T f[8])
-{
-    const T third = (m.e[0][0].i + m.e[1][1].i + m.e[2][2].i) * T(2.0 / 3.0);
-    f[0] = T(2) * m.e[0][0].i - third;
-    f[1] = T(2) * m.e[1][1].i - third;
-    f[2] = m.e[0][1].r - m.e[1][0].r;
-    f[3] = m.e[0][1].i + m.e[1][0].i;
-    f[4] = m.e[0][2].r - m.e[2][0].r;
-    f[5] = m.e[0][2].i + m.e[2][0].i;
-    f[6] = m.e[1][2].r - m.e[2][1].r;
-    f[7] = m.e[1][2].i + m.e[2][1].i;
-}
-
-// Re tr(F G) of two traceless antihermitian matrices in that form
-// (F22 = -(F00 + F11); F_ji = -conj(F_ij)).
-template <class T> __device__ __forceinline__ T retrace_ta2(const T f[8], const T g[8])
-{
-    T s = f[0] * g[0];
-    s = fma(f[1], g[1], s);
-    s = fma(f[0] + f[1], g[0] + g[1], s);
-    T o = f[2] * g[2];
-#pragma unroll
-    for (int k = 3; k < 8; k++) o = fma(f[k], g[k], o);
-    return -s - T(2) * o;
-}
-
 // SU(3) product from the first two rows of `a` (all that load_rows01 reads):
 // the product's first two rows, then its third rebuilt from them.
 template <bool DB, class T> __device__ __forceinline__ M3<T> su3_product(const M3<T> &a, const M3<T> &b)

@@ -43,9 +43,13 @@ __device__ __forceinline__ M3<T> su3_chain4(const M3<T> &a, const M3<T> &b, cons
     return p;
 }
 
+// F = T(C) of the clover C = L1 - L2 + L3 - L4 of plane (mu, nu) at x, in
+// compact form: T is linear, so F = T(L1) - T(L2) + T(L3) - T(L4), and each
+// leaf's last product forms only the entries T needs.  Accumulating 8 reals
+// instead of a 3 x 3 complex C keeps fewer registers live.  plaq = Re tr L1.
 template <class T, int R>
-__device__ __forceinline__ M3<T> clover(const typename vec2<T>::type *__restrict__ f, const Geom &g, const Coord &x,
-                                        int mu, int nu, T &plaq)
+__device__ __forceinline__ void clover(const typename vec2<T>::type *__restrict__ f, const Geom &g, const Coord &x,
+                                       int mu, int nu, T &plaq, T F[8])
 {
     const Coord xpm = step(g, x, mu, +1), xpn = step(g, x, nu, +1);
     const Coord xmm = step(g, x, mu, -1), xmn = step(g, x, nu, -1);
@@ -54,44 +58,60 @@ __device__ __forceinline__ M3<T> clover(const typename vec2<T>::type *__restrict
     const DirView<T> vmu = dir_view<T>(f, g, mu), vnu = dir_view<T>(f, g, nu);
     const M3<T> unu = load_link<T, R>(vnu, ix);
     const int immmn = index_of(g, xmmmn);
+    T t[8];
     if constexpr (R == 2) {
         const M3<T> unu_mn = load_link<T, R>(vnu, imn);
         const M3<T> umu_mm = load_link<T, R>(vmu, imm);
         const M3<T> umu = load_rows01<T, R>(vmu, ix);  // a leading factor only: two rows
         // L1 = U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
-        M3<T> c = su3_chain4<false, false, true, true, R>(umu, load_link<T, R>(vnu, index_of(g, xpm)),
-                                                          load_link<T, R>(vmu, index_of(g, xpn)), unu);
-        plaq = retrace(c);
+        {
+            const M3<T> c = su3_chain4<false, false, true, true, R>(umu, load_link<T, R>(vnu, index_of(g, xpm)),
+                                                                    load_link<T, R>(vmu, index_of(g, xpn)), unu);
+            plaq = retrace(c);
+            ta2_of(c, F);
+        }
         // - L2 = -U_mu(x) U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu)
-        sub_from(c, su3_chain4<false, true, true, false, R>(umu, load_link<T, R>(vnu, index_of(g, xpmmn)),
-                                                            load_link<T, R>(vmu, imn), unu_mn));
+        ta2_of(su3_chain4<false, true, true, false, R>(umu, load_link<T, R>(vnu, index_of(g, xpmmn)),
+                                                       load_link<T, R>(vmu, imn), unu_mn),
+               t);
+#pragma unroll
+        for (int k = 0; k < 8; k++) F[k] -= t[k];
         // + L3 = U_mu(x-mu)^dag U_nu(x-mu-nu)^dag U_mu(x-mu-nu) U_nu(x-nu)
-        add_to(c, su3_chain4<true, true, false, false, R>(umu_mm, load_link<T, R>(vnu, immmn),
-                                                          load_link<T, R>(vmu, immmn), unu_mn));
+        ta2_of(su3_chain4<true, true, false, false, R>(umu_mm, load_link<T, R>(vnu, immmn),
+                                                       load_link<T, R>(vmu, immmn), unu_mn),
+               t);
+#pragma unroll
+        for (int k = 0; k < 8; k++) F[k] += t[k];
         // - L4 = -U_mu(x-mu)^dag U_nu(x-mu) U_mu(x-mu+nu) U_nu(x)^dag
-        sub_from(c, su3_chain4<true, false, false, true, R>(umu_mm, load_link<T, R>(vnu, imm),
-                                                            load_link<T, R>(vmu, index_of(g, xmmpn)), unu));
-        return c;
+        ta2_of(su3_chain4<true, false, false, true, R>(umu_mm, load_link<T, R>(vnu, imm),
+                                                       load_link<T, R>(vmu, index_of(g, xmmpn)), unu),
+               t);
+#pragma unroll
+        for (int k = 0; k < 8; k++) F[k] -= t[k];
     } else {
-        // full products, the last one of each leaf accumulated into C
+        // full products (three stored rows may hold GLAC's det = -1 links)
         const M3<T> umu = load_link<T, R>(vmu, ix);
-        M3<T> c = mul_ad(mul_ad(mul(umu, load_link<T, R>(vnu, index_of(g, xpm))), load_link<T, R>(vmu, index_of(g, xpn))),
-                         unu);
-        plaq = retrace(c);
+        {
+            const M3<T> c =
+                mul_ad(mul_ad(mul(umu, load_link<T, R>(vnu, index_of(g, xpm))), load_link<T, R>(vmu, index_of(g, xpn))),
+                       unu);
+            plaq = retrace(c);
+            ta2_of(c, F);
+        }
         const M3<T> unu_mn = load_link<T, R>(vnu, imn);
-        {
-            M3<T> m = mul_ad(mul_ad(umu, load_link<T, R>(vnu, index_of(g, xpmmn))), load_link<T, R>(vmu, imn));
-            negate(m);
-            mm<false, false, true>(c, m, unu_mn);
-        }
+        ta2_product<false, false>(mul_ad(mul_ad(umu, load_link<T, R>(vnu, index_of(g, xpmmn))), load_link<T, R>(vmu, imn)),
+                                  unu_mn, t);
+#pragma unroll
+        for (int k = 0; k < 8; k++) F[k] -= t[k];
         const M3<T> umu_mm = load_link<T, R>(vmu, imm);
-        mm<false, false, true>(c, mul(mul_dd(umu_mm, load_link<T, R>(vnu, immmn)), load_link<T, R>(vmu, immmn)), unu_mn);
-        {
-            M3<T> m = mul(mul_da(umu_mm, load_link<T, R>(vnu, imm)), load_link<T, R>(vmu, index_of(g, xmmpn)));
-            negate(m);
-            mm<false, true, true>(c, m, unu);
-        }
-        return c;
+        ta2_product<false, false>(mul(mul_dd(umu_mm, load_link<T, R>(vnu, immmn)), load_link<T, R>(vmu, immmn)), unu_mn,
+                                  t);
+#pragma unroll
+        for (int k = 0; k < 8; k++) F[k] += t[k];
+        ta2_product<false, true>(mul(mul_da(umu_mm, load_link<T, R>(vnu, imm)), load_link<T, R>(vmu, index_of(g, xmmpn))),
+                                 unu, t);
+#pragma unroll
+        for (int k = 0; k < 8; k++) F[k] -= t[k];
     }
 }
 
@@ -153,8 +173,17 @@ template <class T> __device__ __forceinline__ M3<T> lds_get(int slot, int lane)
 // added through LDS in the order nu = 1, 2, 3.
 enum MeasMode { MEAS_PLAIN = 0, MEAS_SUPER = 1, MEAS_DENSITY = 2 };
 
+// Waves per SIMD the clover kernel is compiled for (build-time tuning): FP64
+// needs ~250 VGPRs (two waves); FP32 takes ~145 at three waves, and four
+// (128 VGPRs, a few spilled) measured no faster.
+#ifndef GLACX_MEAS_FP32_WAVES
+#define GLACX_MEAS_FP32_WAVES 3
+#endif
+template <class T> constexpr int meas_waves() { return sizeof(T) == 8 ? 2 : GLACX_MEAS_FP32_WAVES; }
+
 template <class T, int R, int MODE>
-__global__ __launch_bounds__(kMeasSites * kMeasWaves, sizeof(T) == 8 ? 2 : 1) void measure_kernel(const typename vec2<T>::type *__restrict__ f,
+__global__ __launch_bounds__(kMeasSites * kMeasWaves)
+__attribute__((amdgpu_waves_per_eu(meas_waves<T>(), meas_waves<T>()))) void measure_kernel(const typename vec2<T>::type *__restrict__ f,
                                                                          Geom g, int cps, double *__restrict__ partials,
                                                                          double *__restrict__ density)
 {
@@ -184,14 +213,15 @@ __global__ __launch_bounds__(kMeasSites * kMeasWaves, sizeof(T) == 8 ? 2 : 1) vo
         const Coord x = decode(g, run_index(g, blk, lane >> 5, lt, k));
         lex = lex_of(g, x);
         const int rho = nu % 3 + 1, sigma = rho % 3 + 1;
-        T p1, p2;
-        const M3<T> c1 = traceless_antiherm2(clover<T, R>(f, g, x, 0, nu, p1));
-        c2 = traceless_antiherm2(clover<T, R>(f, g, x, rho, sigma, p2));
+        T p1, p2, f1[8], f2[8];
+        clover<T, R>(f, g, x, 0, nu, p1, f1);
+        clover<T, R>(f, g, x, rho, sigma, p2, f2);
         acc[0] = p1 + p2;
-        acc[1] = retrace_mul(c1, c1) + retrace_mul(c2, c2);
-        acc[2] = -retrace_mul(c1, c2);
+        acc[1] = retrace_ta2(f1, f1) + retrace_ta2(f2, f2);
+        acc[2] = -retrace_ta2(f1, f2);
         if constexpr (SUPER) {
-            lds_put(nu - 1, lane, c1);
+            c2 = ta2_matrix(f2);
+            lds_put(nu - 1, lane, ta2_matrix(f1));
             lds_put(2 + nu, lane, c2);
         }
     }

@@ -240,6 +240,72 @@ template <class T> __device__ __forceinline__ M3<T> traceless_antiherm2(const M3
     return r;
 }
 
+// T(M) = traceless(M - M^dagger) of a product or a matrix, in the compact
+// form {Im T00, Im T11, T01, T02, T12} (8 reals: T is traceless
+// antihermitian, so T22 = -i (Im T00 + Im T11) and T_ji = -conj(T_ij)).
+// This is traceless_antiherm2 without the entries it determines; the layout
+// is that of su3alg.
+
+// T(op(A) op(B)), op = dagger where the flag is set (the real diagonal of the
+// product is not needed and not computed).
+template <bool DA, bool DB = false, class T>
+__device__ __forceinline__ void ta2_product(const M3<T> &a, const M3<T> &b, T f[8])
+{
+    M3<T> m;
+    mm<DA, DB, false>(m, a, b);
+    const T third = (m.e[0][0].i + m.e[1][1].i + m.e[2][2].i) * T(2.0 / 3.0);
+    f[0] = T(2) * m.e[0][0].i - third;
+    f[1] = T(2) * m.e[1][1].i - third;
+    f[2] = m.e[0][1].r - m.e[1][0].r;
+    f[3] = m.e[0][1].i + m.e[1][0].i;
+    f[4] = m.e[0][2].r - m.e[2][0].r;
+    f[5] = m.e[0][2].i + m.e[2][0].i;
+    f[6] = m.e[1][2].r - m.e[2][1].r;
+    f[7] = m.e[1][2].i + m.e[2][1].i;
+}
+
+// T(M) from the matrix itself.
+template <class T> __device__ __forceinline__ void ta2_of(const M3<T> &m, T f[8])
+{
+    const T third = (m.e[0][0].i + m.e[1][1].i + m.e[2][2].i) * T(2.0 / 3.0);
+    f[0] = T(2) * m.e[0][0].i - third;
+    f[1] = T(2) * m.e[1][1].i - third;
+    f[2] = m.e[0][1].r - m.e[1][0].r;
+    f[3] = m.e[0][1].i + m.e[1][0].i;
+    f[4] = m.e[0][2].r - m.e[2][0].r;
+    f[5] = m.e[0][2].i + m.e[2][0].i;
+    f[6] = m.e[1][2].r - m.e[2][1].r;
+    f[7] = m.e[1][2].i + m.e[2][1].i;
+}
+
+// The full matrix of a compact T.
+template <class T> __device__ __forceinline__ M3<T> ta2_matrix(const T f[8])
+{
+    M3<T> m;
+    m.e[0][0] = {T(0), f[0]};
+    m.e[1][1] = {T(0), f[1]};
+    m.e[2][2] = {T(0), -(f[0] + f[1])};
+    m.e[0][1] = {f[2], f[3]};
+    m.e[1][0] = {-f[2], f[3]};
+    m.e[0][2] = {f[4], f[5]};
+    m.e[2][0] = {-f[4], f[5]};
+    m.e[1][2] = {f[6], f[7]};
+    m.e[2][1] = {-f[6], f[7]};
+    return m;
+}
+
+// Re tr(F G) of two traceless antihermitian matrices in compact form.
+template <class T> __device__ __forceinline__ T retrace_ta2(const T f[8], const T g[8])
+{
+    T s = f[0] * g[0];
+    s = fma(f[1], g[1], s);
+    s = fma(f[0] + f[1], g[0] + g[1], s);
+    T o = f[2] * g[2];
+#pragma unroll
+    for (int k = 3; k < 8; k++) o = fma(f[k], g[k], o);
+    return -s - T(2) * o;
+}
+
 // Compact su(3) element: traceless antihermitian Y with
 //   Y_00 = i a0, Y_11 = i a1, Y_22 = -i (a0 + a1),
 //   Y_01 = (p[0], p[1]), Y_02 = (p[2], p[3]), Y_12 = (p[4], p[5]),
