@@ -13,7 +13,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
 LATTICES = [(2, 2, 2, 2), (4, 4, 4, 4), (4, 4, 4, 8), (6, 4, 8, 6), (6, 6, 6, 6), (2, 6, 4, 10),
-            (8, 8, 8, 16), (12, 8, 4, 6), (24, 24, 24, 48), (32, 32, 32, 32)]
+            (8, 8, 8, 16), (12, 8, 4, 6), (24, 24, 24, 48), (32, 32, 32, 32),
+            # runs of whole 32-site chunks with rows of 2, 6 and 64 entries
+            (4, 16, 4, 4), (12, 16, 4, 8), (128, 2, 4, 4)]
 
 
 @pytest.mark.skipif(not os.path.exists(HIPCC), reason="needs hipcc")
