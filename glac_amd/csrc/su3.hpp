@@ -221,30 +221,12 @@ template <class To, class From> __device__ __forceinline__ M3<To> convert(const 
     return c;
 }
 
-// Traceless antihermitian part of C - C^dagger, i.e. C <- (C - C^dag) - 1/3 tr(C - C^dag).
-// (GLAC/observables/mastersamplertopcxyz.cpp:331-340)
-template <class T> __device__ __forceinline__ M3<T> traceless_antiherm2(const M3<T> &c)
-{
-    M3<T> r;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            r.e[i][j].r = c.e[i][j].r - c.e[j][i].r;
-            r.e[i][j].i = c.e[i][j].i + c.e[j][i].i;
-        }
-    T third = (r.e[0][0].i + r.e[1][1].i + r.e[2][2].i) * T(1.0 / 3.0);
-    r.e[0][0].i -= third;
-    r.e[1][1].i -= third;
-    r.e[2][2].i -= third;
-    return r;
-}
-
-// T(M) = traceless(M - M^dagger) of a product or a matrix, in the compact
-// form {Im T00, Im T11, T01, T02, T12} (8 reals: T is traceless
-// antihermitian, so T22 = -i (Im T00 + Im T11) and T_ji = -conj(T_ij)).
-// This is traceless_antiherm2 without the entries it determines; the layout
-// is that of su3alg.
+// T(M) = (M - M^dagger) - 1/3 tr(M - M^dagger), the traceless antihermitian
+// part the clover observables take of C (GLAC/observables/
+// mastersamplertopcxyz.cpp:331-340), of a product or a matrix, in the
+// compact form {Im T00, Im T11, T01, T02, T12} (8 reals: T is traceless
+// antihermitian, so T22 = -i (Im T00 + Im T11) and T_ji = -conj(T_ij)); the
+// layout is that of su3alg.
 
 // T(op(A) op(B)), op = dagger where the flag is set (the real diagonal of the
 // product is not needed and not computed).
