@@ -216,6 +216,9 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : (MEAS ? 2
     //                                 + U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu) ]
     // (GLAC/actions/wilsongaugeaction.cpp:29-53)
     M3<T> a = zero3<T>();
+    // measuring stage with three stored rows: T(U_mu(x) A_mu(x)) in compact form
+    constexpr bool kCompactForce = MEAS && R == 3;
+    T tz[8] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
     {
 #pragma unroll 1
         for (int k = 0; k < 3; k++) {
@@ -255,24 +258,27 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : (MEAS ? 2
                     // products.  The upper term is finished (its leaf's
                     // plaquette and T(U_mu up)) before the lower one is
                     // formed, which keeps fewer matrices live: T(U_mu (up -
-                    // low)) = T(U_mu up) - T(U_mu low).
+                    // low)) = T(U_mu up) - T(U_mu low).  The force needs only
+                    // T(U_mu A) = sum of T(U_mu up) + T(U_mu low) over the
+                    // terms, so it is accumulated in that compact form (8
+                    // reals instead of the 3 x 3 staple sum).
                     mm<false, true, false>(up, mul_ad(load_link<T, R>(vnu, ipm), load_link<T, R>(vmu, ipn)),
                                            lds_get(sh, nu, lane));
-                    add_to(a, up);
-#pragma unroll
-                    for (int j = 0; j < 8; j++) f[j] = T(0);
-                    if (first) {
+                    T tu[8];
+                    {
                         const M3<T> u = lds_get(sh, mu, lane);
-                        plaq += retrace_mul(u, up);
-                        ta2_product<false>(u, up, f);
+                        if (first) plaq += retrace_mul(u, up);
+                        ta2_product<false>(u, up, tu);
                     }
                     mm<false, false, false>(low, mul_dd(load_link<T, R>(vnu, ipmmn), load_link<T, R>(vmu, imn)),
                                             back_link(nu));
-                    add_to(a, low);
                     T fl[8];
                     ta2_product<false>(lds_get(sh, mu, lane), low, fl);
 #pragma unroll
-                    for (int j = 0; j < 8; j++) f[j] = first ? f[j] - fl[j] : fl[j];
+                    for (int j = 0; j < 8; j++) {
+                        tz[j] += tu[j] + fl[j];
+                        f[j] = first ? tu[j] - fl[j] : fl[j];
+                    }
                 }
                 if ((kPhaseL3 >> e) & 1) {
                     // L3 = (U_d2(y) U_d1(x-d1))^dag (U_d1(y) U_d2(x-d2)),  y = x - d1 - d2
@@ -336,7 +342,16 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? (MEAS ? 3 : 4) : (MEAS ? 2
     }
     if (valid) {
         const M3<T> u = lds_get(sh, mu, lane);
-        const su3alg<T> z = force_from_omega(mul(u, a));
+        su3alg<T> z;
+        if constexpr (kCompactForce) {
+            // Z = -1/2 T(U A) (the form force_from_omega evaluates)
+            z.a0 = T(-0.5) * tz[0];
+            z.a1 = T(-0.5) * tz[1];
+#pragma unroll
+            for (int k = 0; k < 6; k++) z.p[k] = T(-0.5) * tz[2 + k];
+        } else {
+            z = force_from_omega(mul(u, a));
+        }
         su3alg<T> y;
         if (STAGE == 0) {
             acc_store(acc, g, mu, idx, z);
