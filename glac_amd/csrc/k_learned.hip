@@ -176,8 +176,13 @@ __global__ __launch_bounds__(kBlock) void train_kernel(const float *__restrict__
 #pragma unroll 1
         for (int k = 0; k < draws; k++) {
             float z[8], w[8];
+            if (zin) {
 #pragma unroll
-            for (int a = 0; a < 8; a++) z[a] = zin ? zin[((size_t)i * draws + k) * 8 + a] : sym_draw(rs);
+                for (int a = 0; a < 8; a++) z[a] = zin[((size_t)i * draws + k) * 8 + a];
+            } else {
+#pragma unroll
+                for (int a = 0; a < 8; a += 2) sym_draw_pair(rs, z[a], z[a + 1]);
+            }
             float w2 = 0.f;
 #pragma unroll
             for (int a = 0; a < 8; a++) {
@@ -348,7 +353,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 3 : 1) void metropolis_lea
             if (learned) {
                 float z[8], om[8];
 #pragma unroll
-                for (int k = 0; k < 8; k++) z[k] = sym_draw(rs);
+                for (int k = 0; k < 8; k += 2) sym_draw_pair(rs, z[k], z[k + 1]);
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
                     float v = 0.f;

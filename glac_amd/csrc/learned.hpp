@@ -8,8 +8,8 @@
 //   MLP 6 -> 32 -> 32 -> 36 (leaky ReLU, slope 0.01), weights and layer
 //       inputs in bf16, FP32 accumulation (matrix cores, mlp_forward_mfma);
 //   output o -> lower-triangular L: L_aa = exp(o_a), L_ab = o[8 + a(a-1)/2 + b];
-//   per hit w = L z with z symmetric (z_a = (c1 + c2) sqrt 6, c uniform on the
-//       2^16 midpoints of [-1/2, 1/2), two per 32-bit output), X = exp(i sum_a w_a lambda_a / 2),
+//   per hit w = L z with z symmetric (z_a = c sqrt 12, c uniform on the 2^16
+//       midpoints of [-1/2, 1/2), two per 32-bit output), X = exp(i sum_a w_a lambda_a / 2),
 //       U' = X U, accepted with GLAC's exp(-dS) > u.
 // L depends on A only and z -> -z is a symmetry of the draw, so the
 // proposal density of X equals that of X^dagger: exact accept/reject keeps
@@ -233,15 +233,17 @@ template <class T> __device__ __forceinline__ su3alg<T> algebra_from_omega(const
     return y;
 }
 
-// symmetric unit-variance draw from one 32-bit stream output: (c1 + c2) sqrt(6)
-// with c1 (high 16 bits) and c2 (low 16 bits) on the 2^16 midpoints of
-// [-1/2, 1/2), exact in FP32
-template <class S> __device__ __forceinline__ float sym_draw(S &rs)
+// two symmetric unit-variance draws from one 32-bit stream output: each
+// 16-bit half (low half first) on the 2^16 midpoints of [-1/2, 1/2) (exact
+// in FP32), times sqrt(12).  Bin k and bin 65535 - k give exactly opposite
+// values, so z and -z are equally likely.
+template <class S> __device__ __forceinline__ void sym_draw_pair(S &rs, float &z0, float &z1)
 {
     const uint32_t b = rs.next();
+    const float c0 = (float)(b & 0xFFFFu) * 0x1p-16f - (0.5f - 0x1p-17f);
     const float c1 = (float)(b >> 16) * 0x1p-16f - (0.5f - 0x1p-17f);
-    const float c2 = (float)(b & 0xFFFFu) * 0x1p-16f - (0.5f - 0x1p-17f);
-    return (c1 + c2) * 2.4494897427831781f;
+    z0 = c0 * 3.4641016151377546f;
+    z1 = c1 * 3.4641016151377546f;
 }
 
 // Re tr(i lambda_a / 2 N) for a = 0..7

@@ -1111,15 +1111,14 @@ void or_project_algebra(const double *N, double g[8])
     g[7] = -0.5 * r3 * (IM(N, 0, 0) + IM(N, 1, 1) - 2.0 * IM(N, 2, 2));
 }
 
-/* symmetric unit-variance draw from one 32-bit output: (c1 + c2) sqrt(6)
- * with c1 (high 16 bits) and c2 (low 16 bits) uniform on the 2^16 midpoints
- * of [-1/2, 1/2) -- exactly symmetric about 0 */
-static inline double stream_sym(uint32_t s[4])
+/* two symmetric unit-variance draws from one 32-bit output: each 16-bit half
+ * (low half first) uniform on the 2^16 midpoints of [-1/2, 1/2), times
+ * sqrt(12) -- exactly symmetric about 0 */
+static inline void stream_sym_pair(uint32_t s[4], double *z0, double *z1)
 {
     const uint32_t b = or_xoshiro128ss_next(s);
-    const double c1 = (double)(b >> 16) * 0x1p-16 - (0.5 - 0x1p-17);
-    const double c2 = (double)(b & 0xFFFFu) * 0x1p-16 - (0.5 - 0x1p-17);
-    return (c1 + c2) * 2.449489742783178;
+    *z0 = ((double)(b & 0xFFFFu) * 0x1p-16 - (0.5 - 0x1p-17)) * 3.4641016151377546;
+    *z1 = ((double)(b >> 16) * 0x1p-16 - (0.5 - 0x1p-17)) * 3.4641016151377546;
 }
 
 double or_learned_cb_sweep(double *U, const int n[4], double beta, double eps, int nhit, uint64_t seed, uint64_t sweep,
@@ -1148,7 +1147,7 @@ double or_learned_cb_sweep(double *U, const int n[4], double beta, double eps, i
                 for (int h = 0; h < nhit; h++) {
                     if (learned) {
                         double z[8], w[8];
-                        for (int a = 0; a < 8; a++) z[a] = stream_sym(st);
+                        for (int a = 0; a < 8; a += 2) stream_sym_pair(st, &z[a], &z[a + 1]);
                         for (int a = 0; a < 8; a++) {
                             w[a] = 0.0;
                             for (int b = 0; b <= a; b++) w[a] += L[a * 8 + b] * z[b];

@@ -11,7 +11,7 @@ SWEEPS with the learned one, Q and E flowed to t = 1 (50 RK3 steps of 0.02)
 every NCOR sweeps.  <Q^2> and <E(t = 1)> of the two chains agree within
 three autocorrelation-corrected standard errors (Wolff's Gamma method);
 tau_int(Q^2) of both chains is reported (one MI355X run: 9.6 +- 4.1 for
-the fixed-eps chain, 2.0 +- 0.5 for the learned one, in units of 50
+the fixed-eps chain, 2.6 +- 0.7 for the learned one, in units of 50
 sweeps)."""
 import sys
 

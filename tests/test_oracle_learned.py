@@ -88,7 +88,7 @@ def test_learned_single_link_chain_samples_exact_density():
     U = O.to_pairs(np.eye(3))
     vals = []
     for it in range(40000):
-        z = (rng.uniform(-0.5, 0.5, 8) + rng.uniform(-0.5, 0.5, 8)) * np.sqrt(6)
+        z = rng.uniform(-0.5, 0.5, 8) * np.sqrt(12)
         X = O.su3_exp(O.algebra_from_omega(L @ z))
         Up = O.su3_mul(X, U)
         if np.exp(-O.delta_action(U, Up, A, beta)) > rng.uniform():
