@@ -193,6 +193,33 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def learned_sweep_rate(G, n, beta, therm, sweeps, train_steps=3000):
+    """Learned-proposal sweeps per second and the sub-sweep kernel's time,
+    compressed FP32 links: a hot start thermalised by `therm` RST sweeps, the
+    network trained on 20 more (`train_steps` Adam steps), two untimed
+    learned sweeps, then `sweeps` timed ones."""
+    from glac_amd.proposal import train_from_chain
+    with G.Lattice(n, G.FP32_COMPRESSED) as L:
+        L.hot_start(20261202)
+        L.metropolis(therm, beta, 0.24, 10, seed=7)
+        train_from_chain(L, beta, seed=7, sweep_offset=therm, collect_sweeps=20, steps=train_steps)
+        L.metropolis_learned(2, beta, 0.24, 10, seed=8)
+        L.synchronize()
+        L.timing(True)
+        L.timing_reset()
+        t0 = time.perf_counter()
+        acc = L.metropolis_learned(sweeps, beta, 0.24, 10, seed=8, sweep_offset=2)
+        L.synchronize()
+        el = time.perf_counter() - t0
+        launches, ms = L.timing_read(G.K_METROPOLIS_LEARNED)
+        L.timing(False)
+    out = {"lattice": list(n), "sweeps": sweeps, "learned_sweeps_per_s": sweeps / el,
+           "subsweep_kernel_ms": ms / launches, "acceptance": float(acc.mean())}
+    log(f"[bench] learned: {out['learned_sweeps_per_s']:.1f} sweeps/s, sub-sweep {out['subsweep_kernel_ms']:.4f} ms, "
+        f"acceptance {out['acceptance']:.4f}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,6 +241,10 @@ def main():
                     help="run only the independent-configs/s measurement and print its JSON")
     ap.add_argument("--indep-variants", default=",".join(VARIANTS),
                     help="comma-separated variants of the independent-configs/s measurement (%s)" % ", ".join(VARIANTS))
+    ap.add_argument("--only-learned", action="store_true",
+                    help="time only the learned-proposal sweep (compressed FP32 links): thermalise --therm RST "
+                         "sweeps, train on 20 more, then --sweeps learned sweeps; print its JSON")
+    ap.add_argument("--train-steps", type=int, default=3000, help="Adam steps of --only-learned's training")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -227,6 +258,10 @@ def main():
     n = tuple(int(v) for v in args.dims.split(","))
     V = int(np.prod(n))
     beta, eps_flow = 6.1, 0.01
+
+    if args.only_learned:
+        print(json.dumps(learned_sweep_rate(G, n, beta, args.therm, args.sweeps, args.train_steps)), flush=True)
+        return
 
     if args.only_indep:
         variants = tuple(v for v in args.indep_variants.split(",") if v)
