@@ -6,8 +6,10 @@
  * Every call returns 0 on success and a negative GLACX_E_* code on failure;
  * glacx_last_error() then returns the message.  A context owns one HIP stream
  * (default priority) and all device memory; calls on one context must come
- * from one host thread at a time.  Results written through host pointers are
- * complete when the call returns.
+ * from one host thread at a time, and different contexts may be driven from
+ * different threads at once (their streams overlap on the GPU, so several
+ * small lattices fill it together).  Results written through host pointers
+ * are complete when the call returns.
  *
  * Host-side arrays use GLAC's layout: a link is 18 doubles, row-major 3x3
  * complex with (re, im) interleaved (GLAC/math/matrices/su3.h:22,64); a gauge
