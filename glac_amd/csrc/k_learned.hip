@@ -316,15 +316,21 @@ __global__ __launch_bounds__(kBlock) void to_bf16_kernel(const float *__restrict
 }
 
 // ---- sampling with the learned proposal ----------------------------------------
+// Waves per SIMD the FP32 sampling kernel is compiled for (build-time tuning):
+// four fit 128 VGPRs and, with the network's output buffer at 32 floats per
+// link, 38.7 KB of LDS per block.
+#ifndef GLACX_LEARNED_WAVES
+#define GLACX_LEARNED_WAVES 4
+#endif
 template <class T, int R>
-__global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 3 : 1) void metropolis_learned_kernel(typename vec2<T>::type *__restrict__ f, Geom g,
+__global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? GLACX_LEARNED_WAVES : 1) void metropolis_learned_kernel(typename vec2<T>::type *__restrict__ f, Geom g,
                                                                     int mu, int parity, MetroParams prm,
                                                                     const unsigned short *__restrict__ wbf16,
                                                                     const float *__restrict__ mean_std,
                                                                     unsigned long long *__restrict__ count)
 {
     __shared__ __align__(16) MlpImage im;
-    __shared__ __align__(16) float obuf[kBlock / 64][64 * kMlpOut];
+    __shared__ __align__(16) float obuf[kBlock / 64][64 * kMlpObuf];
     mlp_image_fill(im, wbf16, threadIdx.x, kBlock);
     __syncthreads();
     const int h_raw = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
@@ -401,7 +407,7 @@ __global__ __launch_bounds__(kBlock) void proposal_eval_kernel(const float *__re
                                                                float *__restrict__ out, int *__restrict__ ok)
 {
     __shared__ __align__(16) MlpImage im;
-    __shared__ __align__(16) float obuf[kBlock / 64][64 * kMlpOut];
+    __shared__ __align__(16) float obuf[kBlock / 64][64 * kMlpObuf];
     mlp_image_fill(im, wbf16, threadIdx.x, kBlock);
     __syncthreads();
     const int i_raw = blockIdx.x * kBlock + threadIdx.x;

@@ -24,6 +24,9 @@
 namespace glacx {
 
 constexpr int kMlpIn = 6, kMlpH1 = 32, kMlpH2 = 32, kMlpOut = 36;
+// outputs per link that pass through a wavefront's LDS buffer in the matrix-core
+// forward pass (the other four move between lanes by shuffles)
+constexpr int kMlpObuf = 32;
 constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kMlpH1 * kMlpIn, kOffW2 = kOffB1 + kMlpH1,
               kOffB2 = kOffW2 + kMlpH2 * kMlpH1, kOffW3 = kOffB2 + kMlpH2, kOffB3 = kOffW3 + kMlpOut * kMlpH2,
               kNParam = kOffB3 + kMlpOut;
@@ -146,7 +149,7 @@ __device__ __forceinline__ f32x4 lds_bias(const unsigned short *p)
 
 // Network outputs o[36] of this lane's link; every lane of the wavefront must
 // call it (the GEMMs span the wavefront).  im: the parameter image (LDS),
-// obuf: this wavefront's 64 x 36 floats of LDS (16-byte aligned).  Returns
+// obuf: this wavefront's 64 x 32 floats of LDS (16-byte aligned).  Returns
 // false when a standardised feature is out of range.
 __device__ __forceinline__ bool mlp_forward_mfma(const MlpImage &im, const float *mean, const float *stdv,
                                                  const float f[6], float *obuf, float o[36])
@@ -189,15 +192,25 @@ __device__ __forceinline__ bool mlp_forward_mfma(const MlpImage &im, const float
                                                                 lds_bias(&im.b3[16 * m + 4 * grp]), 0, 0, 0);
             d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(lds_frag(&im.w3[16 * m + row][16 + 4 * grp]), a2[1], d, 0, 0, 0);
             const int r0 = 16 * m + 4 * grp;
-            if (r0 < kMlpOut) *reinterpret_cast<f32x4 *>(obuf + (16 * nt + row) * kMlpOut + r0) = d;
+            if (m < 2) {
+                *reinterpret_cast<f32x4 *>(obuf + (16 * nt + row) * kMlpObuf + r0) = d;
+            } else {
+                // outputs 32..35 of link 16 nt + row sit in lane row (grp 0):
+                // the link's own lane takes them by shuffles
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const float v = __shfl(d[k], lane & 15, 64);
+                    if ((lane >> 4) == nt) o[kMlpObuf + k] = v;
+                }
+            }
         }
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the wavefront's LDS stores have landed
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
-    for (int i = 0; i < kMlpOut; i += 4) {
-        const f32x4 v = *reinterpret_cast<const f32x4 *>(obuf + lane * kMlpOut + i);
+    for (int i = 0; i < kMlpObuf; i += 4) {
+        const f32x4 v = *reinterpret_cast<const f32x4 *>(obuf + lane * kMlpObuf + i);
         o[i] = v[0];
         o[i + 1] = v[1];
         o[i + 2] = v[2];
