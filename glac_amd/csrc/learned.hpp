@@ -72,7 +72,8 @@ template <class T> __device__ __forceinline__ void staple_features(const M3<T> &
 // B[4 (l / 16) + 0..3][l % 16] and D[4 (l / 16) + 0..3][l % 16], so the D
 // tile of one layer is, element for element, the B fragment of the next:
 // only the input features (one link per lane, moved with lane shuffles) and
-// the outputs (back to one link per lane, through LDS) change layout.  Layer
+// the outputs (back to one link per lane: 32 through LDS, the last four by
+// lane shuffles) change layout.  Layer
 // inputs are rounded to bf16 (round to nearest even), as the weights are.
 using bf16x4 = short __attribute__((ext_vector_type(4)));
 using f32x4 = float __attribute__((ext_vector_type(4)));
