@@ -248,8 +248,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the lattice shards by 4D domain decomposition, which is not built here).",
+        print(f"bench.py: the library runs a lattice on one GPU; --gpus {args.gpus} is not supported "
+              "(sharding the lattice by 4D domain decomposition is not implemented).",
               file=sys.stderr)
         sys.exit(2)
 
