@@ -143,3 +143,67 @@ def test_mlp_rounds_layer_inputs_to_bf16():
         return float(((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32).view(np.float32)[0])
     assert o[1] == bf16(0.01 * bf16(-0.01))
     assert o[1] != 0.01 * -0.01
+
+
+def test_training_acceptance_model_predicts_the_plain_samplers_outcomes():
+    """The training objective weights each step by alpha_2 = min(1,
+    exp(-dS_2)), dS_2 the second-order expansion of dS in the step's
+    generator (oracle/training.py).  Checked against the accept/reject
+    outcomes the plain sampler records: GLAC's RST proposals (ε = 0.24, 10
+    hits per link; GLAC/system.cpp:333-361) on a thermalised 4^4 field at
+    beta = 6, each proposal X evaluated with its generator Y = log X and the
+    link's W = U A.  The model's mean acceptance equals the recorded
+    acceptance rate, and binned by alpha_2 the recorded rate follows the
+    model: within 0.03 in every decile (about 0.02 below it where
+    acceptance is near 1, the expansion's third-order error).  About seven in
+    ten RST proposals lie far from the identity (GLAC keeps sign(r0), so an
+    SU(2) factor may sit near -1): there dS and dS_2 are both large and both
+    reject."""
+    n = (4, 4, 4, 4)
+    beta, eps = 6.0, 0.24
+    U = O.hot_start_haar(n, 3)
+    for s in range(30):
+        O.cb_sweep(U, n, beta, eps, 10, 3, s)
+    rng = np.random.default_rng(7)
+    rec_A, rec_U, rec_X, acc = [], [], [], []
+    # GLAC's order: x outermost, then y, z, t, the four links of a site
+    # innermost, 10 hits each
+    sites = [(x, y, z, t) for x in range(n[0]) for y in range(n[1]) for z in range(n[2]) for t in range(n[3])]
+    for _sweep in range(4):
+        for x, y, z, t in sites:
+            for mu in range(4):
+                s = x + n[0] * (y + n[1] * (z + n[2] * t))
+                A = O.staple(U, n, x, y, z, t, mu)
+                u = U[mu, s].copy()
+                for _ in range(10):
+                    X = O.generate_rst(rng.uniform(-0.5, 0.5, 13), eps)
+                    up = O.su3_mul(X, u)
+                    accepted = np.exp(-O.delta_action(u, up, A, beta)) > rng.uniform()
+                    rec_A.append(A)
+                    rec_U.append(u.copy())
+                    rec_X.append(X)
+                    acc.append(accepted)
+                    if accepted:
+                        u = up
+                U[mu, s] = u
+    A = O.to_complex(np.array(rec_A))
+    Uc = O.to_complex(np.array(rec_U))
+    X = O.to_complex(np.array(rec_X))
+    acc = np.array(acc, dtype=float)
+    # generator of each proposal: X = V diag(exp(i theta)) V^-1, Y = V diag(i theta) V^-1
+    lam, vec = np.linalg.eig(X)
+    Y = vec @ (1j * np.angle(lam)[..., None] * np.linalg.inv(vec))
+    W = Uc @ A
+    lin = np.trace(Y @ W, axis1=-2, axis2=-1).real
+    quad = np.trace(Y @ Y @ W, axis1=-2, axis2=-1).real
+    alpha2 = np.minimum(1.0, np.exp((beta / 3.0) * (lin + 0.5 * quad)))
+    sigma = np.sqrt(acc.mean() * (1 - acc.mean()) / acc.size)
+    assert abs(alpha2.mean() - acc.mean()) < 3 * sigma, (alpha2.mean(), acc.mean(), sigma)
+    edges = np.quantile(alpha2, np.linspace(0, 1, 11))
+    for i in range(10):
+        m = (alpha2 >= edges[i]) & (alpha2 <= edges[i + 1])
+        print(f"decile {i}: model {alpha2[m].mean():.4f}, recorded {acc[m].mean():.4f}")
+        assert abs(alpha2[m].mean() - acc[m].mean()) < 0.03, (i, alpha2[m].mean(), acc[m].mean())
+    print(f"proposals {acc.size}: recorded acceptance {acc.mean():.4f} +- {sigma:.4f}, model {alpha2.mean():.4f}; "
+          f"alpha_2 < 1e-6 for {np.mean(alpha2 < 1e-6):.3f}")
+    assert 0.6 < np.mean(alpha2 < 1e-6) < 0.8      # the proposals far from the identity
