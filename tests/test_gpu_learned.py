@@ -57,9 +57,18 @@ def test_train_gradient_matches_reference():
     assert np.abs(g - want_g).max() < 2e-3 * scale, np.abs(g - want_g).max() / scale
 
 
-@pytest.mark.parametrize("prec", [glacx.FP64, glacx.FP32_LINKS, glacx.FP32_COMPRESSED], ids=["fp64", "fp32", "fp32c"])
-def test_learned_sweep_follows_oracle_streams(prec):
-    n = (4, 4, 4, 8)
+@pytest.mark.parametrize("n, prec", [((4, 4, 4, 8), glacx.FP64), ((4, 4, 4, 8), glacx.FP32_LINKS),
+                                     ((4, 4, 4, 8), glacx.FP32_COMPRESSED), ((6, 6, 6, 6), glacx.FP64),
+                                     ((6, 6, 6, 6), glacx.FP32_LINKS)],
+                         ids=["4x4x4x8-fp64", "4x4x4x8-fp32", "4x4x4x8-fp32c", "6x6x6x6-fp64", "6x6x6x6-fp32"])
+def test_learned_sweep_follows_oracle_streams(n, prec):
+    """(6, 6, 6, 6): V / 2 = 648 links per sub-sweep, so the last 256-thread
+    block has 136 real lanes and 120 idle ones, which take part in the
+    network's matrix-core GEMMs with a mirrored link and store nothing.  The
+    idle-lane code is common to all stored-row modes; the compressed one,
+    whose two-row products round differently, flips about one decision in
+    10^4 of the 51 840 there (each flip then changes the staples of links
+    updated after it), so it is held to the oracle's decisions at 4^3 x 8."""
     banner(f"learned sweep vs oracle {n} prec={prec}")
     rng = np.random.default_rng(11)
     p = T.init_params(rng, scale0=0.3)
@@ -81,7 +90,7 @@ def test_learned_sweep_follows_oracle_streams(prec):
         O.cb_sweep(U, n, 6.0, 0.24, 10, 3, s)
     # standardisation from the oracle field's staples
     f = np.array([O.staple_features(O.staple(U, n, x, 1, 2, t, mu), 6.0)
-                  for x in range(4) for t in range(8) for mu in range(4)])
+                  for x in range(n[0]) for t in range(n[3]) for mu in range(4)])
     mean, std = f.mean(0).astype(np.float32), (f.std(0) + 1e-3).astype(np.float32)
     W = U.copy()
     acc = O.learned_cb_sweep(W, n, 6.0, 10, 21, 0, pb.astype(np.float64), mean.astype(np.float64),
